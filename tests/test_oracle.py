@@ -1,0 +1,354 @@
+"""Pins for the numpy oracle (oracle/cf_oracle.py).
+
+The reference ships no tests, fixtures or golden vectors for this path, and it
+cannot be compiled here (FFTW 2.1.5 is absent), so each oracle function is
+checked against an independent statement of the same mathematics: numpy's
+Chebyshev module, brute-force sums, numpy.linalg, eigenvalue problems and the
+Orr-Sommerfeld eigenvalue of Orszag (1971).
+"""
+import numpy as np
+import numpy.polynomial.chebyshev as C
+import pytest
+
+from oracle import cf_oracle as O
+
+
+def small_params(nnx=8, nny=8, nnz=6, aphi=1.0, beta=0.5, dt=1e-3, re=100.0, re_p=0.0):
+    return O.Params(nnx=nnx, nny=nny, nnz=nnz, aphi=aphi, beta=beta, dt=dt, re=re, re_p=re_p)
+
+
+def rand_field(p, rng, scale=1.0):
+    return scale * (rng.standard_normal((p.Ny, p.Mz, p.Mx)) + 1j * rng.standard_normal((p.Ny, p.Mz, p.Mx)))
+
+
+# ---------------------------------------------------------------- SCHEB / dz
+
+
+@pytest.mark.parametrize("n", [8, 17, 32])
+def test_scheb_matches_chebyshev_derivatives(n):
+    t0, t1, t2, t4 = O.scheb(n)
+    y = np.cos(np.pi * np.arange(n + 1) / n)
+    for l in range(n + 1):
+        c = np.zeros(n + 1)
+        c[l] = 1.0
+        ref = [C.chebval(y, C.chebder(c, m)) for m in (0, 1, 2, 4)]
+        np.testing.assert_allclose(t0[:, l], ref[0], atol=1e-12)
+        scale1 = max(1.0, l ** 2)
+        np.testing.assert_allclose(t1[:, l], ref[1], atol=1e-11 * scale1)
+        scale2 = max(1.0, l ** 4)
+        np.testing.assert_allclose(t2[:, l], ref[2], atol=1e-10 * scale2)
+        # the closed form divides by sin^7(q): near-wall rounding ~ eps (n/pi)^7 l^2
+        tol4 = max(1e-9 * l ** 8, 1e-14 * (n / np.pi) ** 7 * max(1, l) ** 2)
+        np.testing.assert_allclose(t4[1:n, l], ref[3][1:n], atol=tol4, rtol=1e-7)
+
+
+def test_dz_matches_chebder():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 7, 32):
+        a = rng.standard_normal((n + 1, 3))
+        got = O.dz(a)
+        for c in range(3):
+            ref = np.zeros(n + 1)
+            d = C.chebder(a[:, c])
+            ref[:len(d)] = d
+            np.testing.assert_allclose(got[:, c], ref, atol=1e-12 * max(1, n ** 2))
+
+
+# ------------------------------------------------------- padded Fourier pair
+
+
+def brute_phys(f, p):
+    """phys(z_n, x_m) = Re(sum_j F(0,j) e^{i kz z}) + sum_{i>0} 2 Re(sum_j F(i,j) e^{i(kx x + kz z)})."""
+    h = p.Mz // 2
+    jj = np.arange(p.Mz)
+    kz = np.where(jj <= h, jj, jj - p.Mz)
+    keep = jj != h
+    zn = 2 * np.pi * np.arange(p.Lz) / p.Lz
+    xm = 2 * np.pi * np.arange(p.Lx) / p.Lx
+    ez = np.exp(1j * kz[None, :] * zn[:, None]) * keep[None, :]       # (Lz, Mz)
+    ex = np.exp(1j * np.arange(p.Mx)[None, :] * xm[:, None])          # (Lx, Mx)
+    hz = np.einsum("nj,kji->kni", ez, f)                               # (K, Lz, Mx)
+    out = hz[..., 0:1].real + 0 * xm[None, None, :]
+    out = out + 2 * np.einsum("kni,mi->knm", hz[..., 1:], ex[:, 1:]).real
+    return out
+
+
+def test_fourier_to_phys_brute_force():
+    p = small_params(nnx=8, nny=8, nnz=2)
+    rng = np.random.default_rng(5)
+    f = rand_field(p, rng)
+    np.testing.assert_allclose(O.fourier_to_phys(f, p), brute_phys(f, p), atol=1e-12)
+
+
+def hermitian_project(f, p):
+    """What the padded pair keeps: Nyquist column zero, kx=0 plane Hermitian part."""
+    g = f.copy()
+    h = p.Mz // 2
+    g[:, h, :] = 0
+    col = g[:, :, 0].copy()
+    jj = np.arange(p.Mz)
+    neg = (-jj) % p.Mz
+    g[:, :, 0] = 0.5 * (col + np.conj(col[:, neg]))
+    g[:, h, 0] = 0
+    return g
+
+
+def test_phys_roundtrip_on_retained_band():
+    p = small_params(nnx=16, nny=8, nnz=3)
+    rng = np.random.default_rng(6)
+    f = rand_field(p, rng)
+    back = O.phys_to_fourier(O.fourier_to_phys(f, p), p)
+    np.testing.assert_allclose(back, hermitian_project(f, p), atol=1e-13)
+
+
+# ------------------------------------------------ wall-normal padded product
+
+
+@pytest.mark.parametrize("n", [4, 9, 16])
+def test_cheb_padded_product_is_truncated_chebmul(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n + 1, 2))
+    b = rng.standard_normal((n + 1, 2))
+    got = O.cheb_padded_product(a, b, n, shift=0.25)
+    for c in range(2):
+        ref = np.zeros(n + 1)
+        prod = C.chebmul(a[:, c], b[:, c])
+        ref[:] = prod[:n + 1]
+        ref[0] -= 0.25
+        np.testing.assert_allclose(got[:, c], ref, atol=1e-13)
+
+
+def full_spectrum(f, p):
+    """Dict (kx, kz) -> Chebyshev coefficients of the real field the padded pair sees."""
+    g = hermitian_project(f, p)
+    h = p.Mz // 2
+    out = {}
+    for j in range(p.Mz):
+        if j == h:
+            continue
+        kz = j if j < h else j - p.Mz
+        for i in range(p.Mx):
+            out[(i, kz)] = g[:, j, i]
+            if i > 0:
+                out[(-i, -kz)] = np.conj(g[:, j, i])
+    return out
+
+
+def test_getlamb_equals_truncated_convolution():
+    p = small_params(nnx=8, nny=8, nnz=4, re_p=0.3)
+    rng = np.random.default_rng(11)
+    u = rand_field(p, rng)
+    w = rand_field(p, rng)
+    got = O.getlamb(u, w, p, shift=p.re_p)
+    su, sw = full_spectrum(u, p), full_spectrum(w, p)
+    h = p.Mz // 2
+    for j in range(p.Mz):
+        kz = j if j <= h else j - p.Mz
+        for i in range(p.Mx):
+            acc = np.zeros(2 * p.N + 1, dtype=complex)
+            for (ax, az), ca in su.items():
+                key = (i - ax, kz - az)
+                if key in sw:
+                    cb = sw[key]
+                    acc += C.chebmul(ca.real, cb.real) - C.chebmul(ca.imag, cb.imag) \
+                        + 1j * (C.chebmul(ca.real, cb.imag) + C.chebmul(ca.imag, cb.real))
+            ref = acc[:p.N + 1]
+            if i == 0 and j == 0:
+                ref = ref.copy()
+                ref[0] -= p.re_p
+            np.testing.assert_allclose(got[:, j, i], ref, atol=1e-11, err_msg=f"mode {(i, j)}")
+
+
+def test_points_from_coeffs():
+    n = 12
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((n + 1, 4))
+    y = np.cos(np.pi * np.arange(n + 1) / n)
+    np.testing.assert_allclose(O.points_from_coeffs(a, n), C.chebval(y, a).T, atol=1e-12)
+
+
+# ------------------------------------------------------------- operators
+
+
+def test_operator_inverse_identity():
+    """A.A^-1 = I: the reference's own start-up check (PF:206-212)."""
+    p = small_params(nnz=24, dt=1e-3, re=3000.0)
+    ops = O.Operators(p)
+    for kmn in (0.25, 1.0, 17.0):
+        for a in (ops.coef_v(kmn), ops.coef_omegay(kmn)):
+            np.testing.assert_allclose(a @ np.linalg.inv(a), np.eye(p.Ny), atol=1e-9)
+
+
+def test_gsij_matches_linear_solve():
+    rng = np.random.default_rng(7)
+    for n in (5, 17):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        np.testing.assert_allclose(O.gsij(a, b), np.linalg.solve(a, b), atol=1e-11)
+    p = small_params(nnz=32)
+    ops = O.Operators(p)
+    b = rng.standard_normal(p.Ny) + 0j
+    np.testing.assert_allclose(O.gsij(ops.coef_u0(), b), np.linalg.solve(ops.coef_u0(), b), atol=1e-10)
+
+
+# ------------------------------------------------------------ time stepping
+
+
+def laminar_state(p, u_c):
+    au0 = np.zeros(p.Ny, complex)
+    au0[0], au0[2] = 0.5 * u_c, -0.5 * u_c
+    z = np.zeros((p.Ny, p.Mz, p.Mx), complex)
+    return O.initial_presolve_state(z, z, au0, np.zeros(p.Ny, complex), p)
+
+
+def test_laminar_profile_is_a_fixed_point():
+    """U = (re re_p / 2)(1 - y^2) is an exact discrete steady state once the
+    AB2 history holds the steady forcing (the cold start sets L^{-1} = 0,
+    PF:2316, so the first step from initial() is not steady)."""
+    re = 3000.0
+    p = small_params(nnx=8, nny=8, nnz=16, re=re, re_p=2.0 / re, dt=1e-3)
+    st = laminar_state(p, 1.0)
+    np.testing.assert_allclose(st["clamb1n"], -p.re_p, atol=1e-15)
+    st["clamb1nm1"] = st["clamb1n"].copy()
+    au0 = st["au0"].copy()
+    O.Stepper(p).step(st, 5)
+    np.testing.assert_allclose(st["au0"], au0, atol=1e-13)
+    for k in ("vnp1", "omegaynp1", "rhsvn", "rhsomegayn"):
+        assert np.max(np.abs(st[k])) < 1e-12, k
+
+
+def collocation_eig(p, kmn, which):
+    """Generalised eigenpairs of the CN pair (A, B) on the solved subspace."""
+    ops = O.Operators(p)
+    if which == "omegay":
+        a, brows, nb = ops.coef_omegay(kmn), ops.rhs_op_omegay(kmn), 2
+    else:
+        a, brows, nb = ops.coef_v(kmn), ops.rhs_op_v(kmn), 4
+    b = np.zeros_like(a)
+    b[nb:] = brows
+    # B x = g A x restricted to vectors satisfying the BC rows: eig of A^-1 B
+    g, vec = np.linalg.eig(np.linalg.inv(a) @ b)
+    return g, vec
+
+
+@pytest.mark.parametrize("which", ["omegay", "v"])
+def test_crank_nicolson_amplification(which):
+    """An eigenvector of A^-1 B is multiplied by its eigenvalue in one step,
+    and that eigenvalue is the CN factor (1 + c lam)/(1 - c lam) of the
+    collocation operator's eigenvalue lam (c = dt/2re)."""
+    p = small_params(nnx=8, nny=8, nnz=20, re=50.0, dt=2e-3)
+    kmn = 1.0 + 4.0   # mode (i=1, j=1): alpha=1, beta=2
+    g, vec = collocation_eig(p, kmn, which)
+    order = np.argsort(-np.abs(g))
+    g = g[order]
+    vec = vec[:, order]
+    # pick the least damped non-trivial eigenpair
+    k = np.argmax(np.abs(g) < 1.0 - 1e-12)
+    mode = vec[:, k] * 1e-9
+    st = O.zero_state(p)
+    key = "omegaynp1" if which == "omegay" else "vnp1"
+    st[key][:, 1, 1] = mode
+    stepper = O.Stepper(p)
+    stepper.solve(st)
+    np.testing.assert_allclose(st[key][:, 1, 1], g[k] * mode, rtol=1e-9, atol=1e-20)
+    # the CN factor relation: g = (1 + c lam)/(1 - c lam)
+    c = p.dt * 0.5 / p.re
+    lam = (g[k] - 1.0) / (c * (g[k] + 1.0))
+    assert lam.real < 0
+
+
+def test_kinematic_identities_after_solve():
+    p = O.Params(nnx=8, nny=8, nnz=16, aphi=1.0, beta=0.5, dt=1e-3, re=500.0, re_p=2.0 / 500.0)
+    rng = np.random.default_rng(9)
+    from channel_flow_amd import ic
+    v, oy, au0, w0 = ic.seeded_ic(p.nnx, p.nny, p.nnz, p.aphi, p.beta, 1.0, 1e-2, 3.0, 0.0, 42)
+    st = O.initial_presolve_state(v, oy, au0, w0, p)
+    stepper = O.Stepper(p)
+    stepper.solve(st)
+    alpha, beta = O.wavenumbers(p)
+    for (i, j, kmn) in stepper.solve_modes:
+        u, vv, w = st["unp1"][:, j, i], st["vnp1"][:, j, i], st["wnp1"][:, j, i]
+        oyy = st["omegaynp1"][:, j, i]
+        a_, b_ = alpha[i], beta[j]
+        div = 1j * a_ * u + O.dz(vv) + 1j * b_ * w
+        assert np.max(np.abs(div)) < 1e-12
+        np.testing.assert_allclose(1j * b_ * u - 1j * a_ * w, oyy, atol=1e-12)
+        np.testing.assert_allclose(st["omegaxnp1"][:, j, i], O.dz(w) - 1j * b_ * vv, atol=1e-12)
+        np.testing.assert_allclose(st["omegaznp1"][:, j, i], 1j * a_ * vv - O.dz(u), atol=1e-12)
+    del rng
+
+
+def orr_sommerfeld_eig(re, alpha, n):
+    """Least stable Orr-Sommerfeld eigenvalue c for plane Poiseuille U = 1 - y^2.
+
+    Chebyshev collocation in coefficient space with v = v' = 0 at y = +-1 (an
+    independent discretisation from the time stepper's operators)."""
+    y = np.cos(np.pi * np.arange(n + 1) / n)
+    T = np.zeros((n + 1, n + 1))
+    D2 = np.zeros_like(T)
+    D4 = np.zeros_like(T)
+    for l in range(n + 1):
+        c = np.zeros(n + 1)
+        c[l] = 1
+        T[:, l] = C.chebval(y, c)
+        D2[:, l] = C.chebval(y, C.chebder(c, 2))
+        D4[:, l] = C.chebval(y, C.chebder(c, 4))
+    U = 1 - y ** 2
+    Upp = -2.0
+    a2 = alpha ** 2
+    L = (D4 - 2 * a2 * D2 + a2 ** 2 * T) / (1j * alpha * re) - (U[:, None] * (D2 - a2 * T) - Upp * T)
+    M = -(D2 - a2 * T)
+    # c: L v = c M v  with rows 0,1,n-1,n replaced by BCs
+    A = L.copy()
+    B = M.copy()
+    bc = np.zeros((4, n + 1))
+    bc[0] = 1.0
+    bc[1] = (-1.0) ** np.arange(n + 1)
+    bc[2] = np.arange(n + 1) ** 2
+    bc[3] = -((-1.0) ** np.arange(n + 1)) * np.arange(n + 1) ** 2
+    for r, row in zip((0, n, 1, n - 1), bc):
+        A[r] = row
+        B[r] = 0
+    import scipy.linalg
+    c, vec = scipy.linalg.eig(A, B)
+    ok = np.isfinite(c)
+    c, vec = c[ok], vec[:, ok]
+    k = np.argmax(c.imag)
+    return c[k], vec[:, k]
+
+
+def test_orr_sommerfeld_solver_matches_orszag():
+    c, _ = orr_sommerfeld_eig(10000.0, 1.0, 80)
+    assert abs(c.real - 0.23752649) < 2e-6
+    assert abs(c.imag - 0.00373967) < 2e-6
+
+
+@pytest.mark.slow
+def test_tollmien_schlichting_growth_through_full_step():
+    """Whole-step physics pin: an infinitesimal 2-D TS wave on U = 1 - y^2 at
+    Re = 10000, alpha = 1 evolves as exp(-i alpha c t) with Orszag's
+    c = 0.23752649 + 0.00373967 i (Orszag 1971).  Exercises the products with
+    the mean flow, the v solve, kinematics and the AB2/CN scheme."""
+    re = 10000.0
+    n = 64
+    dt = 0.01
+    p = O.Params(nnx=8, nny=4, nnz=n, aphi=1.0, beta=0.5, dt=dt, re=re, re_p=2.0 / re)
+    c_ref = 0.23752649 + 0.00373967j
+    _, vhat = orr_sommerfeld_eig(re, 1.0, n)
+    # coefficient-space eigenfunction; real part of v e^{i x} is the field
+    v = np.zeros((p.Ny, p.Mz, p.Mx), complex)
+    v[:, 0, 1] = 1e-7 * vhat / np.max(np.abs(vhat))
+    oy = np.zeros_like(v)
+    au0 = np.zeros(p.Ny, complex)
+    au0[0], au0[2] = 0.5, -0.5
+    st = O.initial_presolve_state(v, oy, au0, np.zeros(p.Ny, complex), p)
+    stepper = O.Stepper(p)
+    amps = []
+    for _ in range(60):
+        stepper.step(st, 1)
+        amps.append(st["vnp1"][:, 0, 1].copy())
+    # ratio of successive amplitudes once the AB2 start-up transient has passed
+    r = np.vdot(amps[-2], amps[-1]) / np.vdot(amps[-2], amps[-2])
+    c_num = 1j * np.log(r) / (1.0 * dt)
+    assert abs(c_num.real - c_ref.real) < 2e-4
+    assert abs(c_num.imag - c_ref.imag) < 2e-4
