@@ -1,0 +1,144 @@
+"""Host-side interface of the GPU timestep, named after the reference call sites.
+
+    ch = Channel(nnx=32, nny=32, nnz=32, aphi=1.0, beta=0.5, dt=1e-3,
+                 re=3000.0, re_p=2/3000)
+    ch.initial(v, omegay, au0, w0)       # initial(isign=-1), PF:1932-1946
+    ch.step(100)                         # 100 x [solve ; getnonlinear], PF:220-532
+    st = ch.get_state()                  # output(), PF:2460-2504
+
+State dictionaries use the reference's array names (PC:28-38) and memory
+layout: complex128[Ny][Mz][Mx] fields (== real*8 (0:nx,0:ny,0:nz) with
+interleaved Re/Im) and complex128[Ny] mean-mode arrays (== real*8 (2,0:nz)).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import runtime as rt
+
+
+class Channel:
+    def __init__(self, nnx, nny, nnz, aphi, beta, dt, re, re_p, precision="fp64"):
+        self._lib = rt.load()
+        prec = {"fp64": 0, "mixed": 1}[precision]
+        self.params = rt.ChfParams(nnx, nny, nnz, aphi, beta, dt, re, re_p, prec)
+        self._ctx = ctypes.c_void_p()
+        rc = self._lib.chf_create(ctypes.byref(self.params), ctypes.byref(self._ctx))
+        if rc != 0:
+            msg = self._lib.chf_last_error(None).decode()
+            raise rt.ChfError(f"chf_create failed ({rc}): {msg}")
+        mx, mz, ny = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._lib.chf_dims(self._ctx, ctypes.byref(mx), ctypes.byref(mz), ctypes.byref(ny))
+        self.Mx, self.Mz, self.Ny = mx.value, mz.value, ny.value
+        self.precision = precision
+
+    # -------------------------------------------------------------- plumbing
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.chf_last_error(self._ctx).decode()
+            raise rt.ChfError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self._ctx:
+            self._lib.chf_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def field_shape(self):
+        return (self.Ny, self.Mz, self.Mx)
+
+    def _as_field(self, a, name):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != self.field_shape:
+            raise ValueError(f"{name}: expected shape {self.field_shape}, got {a.shape}")
+        return a
+
+    def _as_mean(self, a, name):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != (self.Ny,):
+            raise ValueError(f"{name}: expected shape {(self.Ny,)}, got {a.shape}")
+        return a
+
+    @staticmethod
+    def _dp(a):
+        return a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    # ------------------------------------------------------------ interface
+    def set_state(self, st: dict):
+        """Load a pre-solve state (the restart record, PF:1913-1917)."""
+        arrs = {}
+        for n in rt.STATE_FIELDS:
+            arrs[n] = (self._as_mean if n in rt.MEAN_FIELDS else self._as_field)(st[n], n)
+        cs = rt.ChfState(**{n: self._dp(arrs[n]) for n in rt.STATE_FIELDS})
+        self._check(self._lib.chf_set_state(self._ctx, ctypes.byref(cs)), "chf_set_state")
+
+    def get_state(self) -> dict:
+        """Read the pre-solve state (what output writes, PF:2476-2479)."""
+        out = {n: np.empty((self.Ny,) if n in rt.MEAN_FIELDS else self.field_shape, np.complex128)
+               for n in rt.STATE_FIELDS}
+        cs = rt.ChfState(**{n: self._dp(out[n]) for n in rt.STATE_FIELDS})
+        self._check(self._lib.chf_get_state(self._ctx, ctypes.byref(cs)), "chf_get_state")
+        return out
+
+    def initial(self, vnp1, omegaynp1, au0, w0):
+        """Cold start from v, omega_y and the mean profiles (PF:1932-1946)."""
+        v = self._as_field(vnp1, "vnp1")
+        o = self._as_field(omegaynp1, "omegaynp1")
+        a = self._as_mean(au0, "au0")
+        w = self._as_mean(w0, "w0")
+        self._check(self._lib.chf_init_fields(self._ctx, self._dp(v), self._dp(o), self._dp(a),
+                                              self._dp(w)), "chf_init_fields")
+
+    def getnonlinear(self):
+        self._check(self._lib.chf_nonlinear(self._ctx), "chf_nonlinear")
+
+    def solve(self):
+        self._check(self._lib.chf_solve(self._ctx), "chf_solve")
+
+    def step(self, n=1):
+        self._check(self._lib.chf_step(self._ctx, int(n)), "chf_step")
+
+    def sync(self):
+        self._check(self._lib.chf_sync(self._ctx), "chf_sync")
+
+    def fields(self) -> dict:
+        """Post-solve u, w, omega_x, omega_z (PF:387-472)."""
+        out = {n: np.empty(self.field_shape, np.complex128)
+               for n in ("unp1", "wnp1", "omegaxnp1", "omegaznp1")}
+        self._check(self._lib.chf_get_fields(self._ctx, *(self._dp(out[n]) for n in
+                                                          ("unp1", "wnp1", "omegaxnp1", "omegaznp1"))),
+                    "chf_get_fields")
+        return out
+
+    def device_bytes(self) -> int:
+        return int(self._lib.chf_device_bytes(self._ctx))
+
+    # --------------------------------------------------------- instrumentation
+    def set_timing(self, on: bool):
+        """Bracket every kernel launch with HIP events on the context stream."""
+        self._check(self._lib.chf_set_timing(self._ctx, int(bool(on))), "chf_set_timing")
+
+    def kernel_times(self) -> dict:
+        """{kernel class: (total ms, launches)} since the last set_timing."""
+        out = {}
+        for k in range(self._lib.chf_kernel_count()):
+            name = self._lib.chf_kernel_name(k).decode()
+            ms, n = ctypes.c_double(), ctypes.c_int()
+            self._check(self._lib.chf_timing(self._ctx, k, ctypes.byref(ms), ctypes.byref(n)),
+                        "chf_timing")
+            out[name] = (ms.value, n.value)
+        return out

@@ -1,0 +1,502 @@
+// C ABI of libchf: context life cycle, state exchange in the reference
+// layout, and the step sequence of the reference time loop (PF:220-532).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "chf_internal.h"
+
+using namespace chf;
+
+namespace chf {
+const char* const kKernelNames[K_COUNT] = {
+    "synth_padded", "zfft_inv", "xpass", "zfft_fwd", "analysis", "hassemble",
+    "synth_points", "mean_points", "rhs_v", "rhs_o", "solve"};
+}
+
+namespace {
+
+thread_local std::string g_create_err;
+
+#define CHF_TRY(expr)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess) {                                             \
+      c->err = std::string(#expr) + ": " + hipGetErrorString(e_);       \
+      return (int)e_;                                                   \
+    }                                                                   \
+  } while (0)
+
+int fail(Ctx* c, const std::string& msg) {
+  if (c) c->err = msg;
+  return -1;
+}
+
+// SCHEB (PF:3272-3342): T0, T1, T2, T4 at y_j = cos(pi j/N), rows j = points.
+void scheb(int n, std::vector<double>& t0, std::vector<double>& t1, std::vector<double>& t2,
+           std::vector<double>& t4) {
+  const int m = n + 1;
+  t0.assign((size_t)m * m, 0.0);
+  t1.assign((size_t)m * m, 0.0);
+  t2.assign((size_t)m * m, 0.0);
+  t4.assign((size_t)m * m, 0.0);
+  for (int l = 0; l <= n; ++l) {
+    const double L = (double)l;
+    const double sg = (l % 2 == 0) ? 1.0 : -1.0;
+    t0[0 * m + l] = 1.0;
+    t0[(size_t)n * m + l] = sg;
+    t1[0 * m + l] = L * L;
+    t1[(size_t)n * m + l] = -sg * L * L;
+    t2[0 * m + l] = L * L * (L * L - 1.0) / 3.0;
+    t2[(size_t)n * m + l] = sg * L * L * (L * L - 1.0) / 3.0;
+  }
+  const double h0 = 4.0 * std::atan(1.0) / (double)n;
+  for (int j = 1; j < n; ++j) {
+    const double q = h0 * j;
+    const double c1 = std::cos(q), c2 = c1 * c1, s1 = std::sin(q), s2 = s1 * s1;
+    const double ds1 = 1.0 / s1, ds3 = ds1 / s2, ds5 = ds3 / s2, ds7 = ds5 / s2;
+    for (int l = 0; l <= n; ++l) {
+      const double bj = (double)l;
+      const double qj = q * bj;
+      const double sj = std::sin(qj), cj = std::cos(qj);
+      const double bs = bj * bj * s2, bc = bj * cj * s1;
+      const size_t o = (size_t)j * m + l;
+      t0[o] = cj;
+      t1[o] = bj * sj * ds1;
+      t2[o] = bj * (sj * c1 - bc) * ds3;
+      t4[o] = bj * (sj * c1 * (9.0 + 6.0 * (c2 - bs)) - bc * (4.0 + 11.0 * c2 - bs)) * ds7;
+    }
+  }
+}
+
+// cos(pi a / b) with the argument reduced exactly in integers.
+double cospi_frac(long a, long b) {
+  const long per = 2 * b;
+  long r = a % per;
+  if (r < 0) r += per;
+  return std::cos(M_PI * (double)r / (double)b);
+}
+
+template <class T>
+hipError_t dalloc(Ctx* c, T** p, size_t n) {
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e == hipSuccess) c->bytes += (long long)(n * sizeof(T));
+  return e;
+}
+
+int validate(const chf_params* p, std::string& msg) {
+  if (!p) { msg = "null parameters"; return -1; }
+  if (p->nnx < 4 || p->nnx % 4) { msg = "nnx must be a positive multiple of 4 (PF:721-765 padding)"; return -1; }
+  if (p->nny < 4 || p->nny % 4) { msg = "nny must be a positive multiple of 4 (PF:721-765 padding)"; return -1; }
+  if (p->nnz < 4 || p->nnz > 256) { msg = "nnz must lie in [4, 256]"; return -1; }
+  if (!fft_length_ok(3 * p->nnx / 2) || !fft_length_ok(3 * p->nny / 2)) {
+    msg = "3*nnx/2 and 3*nny/2 must be of the form 2^a 3^b";
+    return -1;
+  }
+  if (!(p->aphi > 0) || !(p->beta > 0) || !(p->dt > 0) || !(p->re > 0)) {
+    msg = "aphi, beta, dt and re must be positive";
+    return -1;
+  }
+  if (p->precision != 0 && p->precision != 1) { msg = "precision must be 0 (FP64) or 1 (mixed)"; return -1; }
+  if (p->precision == 1) { msg = "mixed precision is not built yet"; return -1; }
+  return 0;
+}
+
+void free_all(Ctx* c) {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(c->alpha); f(c->beta);
+  f(c->v); f(c->oy); f(c->u); f(c->w); f(c->ox); f(c->oz);
+  for (int i = 0; i < 2; ++i) { f(c->hv[i]); f(c->hg[i]); f(c->cl1[i]); f(c->cl3[i]); }
+  f(c->au0); f(c->w0);
+  for (auto& y : c->Y) f(y);
+  for (auto& z : c->Z) f(z);
+  for (auto& l : c->Lc) f(l);
+  f(c->hvc); f(c->hgc);
+  f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
+  f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
+  f(c->invV); f(c->invW); f(c->twx); f(c->twz);
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  c->event_pool.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int build(Ctx* c) {
+  Dims& d = c->d;
+  const chf_params& p = c->prm;
+  d.Mx = p.nnx / 2;
+  d.Mz = p.nny;
+  d.N = p.nnz;
+  d.Ny = p.nnz + 1;
+  d.Lx = 3 * p.nnx / 2;
+  d.Lz = 3 * p.nny / 2;
+  d.Py = 2 * d.N + 1;
+  d.Ph = d.N + 1;
+  d.ne = d.N / 2 + 1;
+  d.no = (d.N + 1) / 2;
+  d.Pn = d.N / 2 + 1;
+  d.ncol = (long)d.Mz * d.Mx;
+  d.ngroups = d.Mx * (d.Mz / 2);
+  d.inv_stride = (long)d.ne * d.ne + (long)d.no * d.no;
+
+  CHF_TRY(hipGetDevice(&c->device));
+  int max_lds = 0;
+  CHF_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+  {
+    // largest LDS request: the set-up inversion of the even half system
+    const long need = 8L * d.ne * d.ne + 2 * 160 * 8;
+    if (need > max_lds) {
+      c->err = "nnz too large for the LDS-resident operator inversion on this device";
+      return -1;
+    }
+  }
+  CHF_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+  const size_t fld = (size_t)d.Ny * d.ncol;
+  CHF_TRY(dalloc(c, &c->alpha, d.Mx));
+  CHF_TRY(dalloc(c, &c->beta, d.Mz));
+  CHF_TRY(dalloc(c, &c->v, fld));
+  CHF_TRY(dalloc(c, &c->oy, fld));
+  CHF_TRY(dalloc(c, &c->u, fld));
+  CHF_TRY(dalloc(c, &c->w, fld));
+  CHF_TRY(dalloc(c, &c->ox, fld));
+  CHF_TRY(dalloc(c, &c->oz, fld));
+  for (int i = 0; i < 2; ++i) {
+    CHF_TRY(dalloc(c, &c->hv[i], fld));
+    CHF_TRY(dalloc(c, &c->hg[i], fld));
+    CHF_TRY(dalloc(c, &c->cl1[i], d.Ny));
+    CHF_TRY(dalloc(c, &c->cl3[i], d.Ny));
+  }
+  CHF_TRY(dalloc(c, &c->au0, d.Ny));
+  CHF_TRY(dalloc(c, &c->w0, d.Ny));
+  const size_t ysz = (size_t)d.Py * d.ncol;
+  const size_t zsz = (size_t)d.Py * d.Lz * d.Mx;
+  for (auto& y : c->Y) CHF_TRY(dalloc(c, &y, ysz));
+  for (auto& z : c->Z) CHF_TRY(dalloc(c, &z, zsz));
+  for (auto& l : c->Lc) CHF_TRY(dalloc(c, &l, fld));
+  CHF_TRY(dalloc(c, &c->hvc, fld));
+  CHF_TRY(dalloc(c, &c->hgc, fld));
+  CHF_TRY(dalloc(c, &c->rvE, (size_t)d.ne * d.ncol));
+  CHF_TRY(dalloc(c, &c->rvO, (size_t)d.ne * d.ncol));
+  CHF_TRY(dalloc(c, &c->roE, (size_t)d.ne * d.ncol));
+  CHF_TRY(dalloc(c, &c->roO, (size_t)d.ne * d.ncol));
+  CHF_TRY(dalloc(c, &c->invV, (size_t)d.ngroups * d.inv_stride));
+  CHF_TRY(dalloc(c, &c->invW, (size_t)d.ngroups * d.inv_stride));
+
+  // wavenumbers (PF:123-134, id = 0)
+  std::vector<double> al(d.Mx), be(d.Mz);
+  for (int i = 0; i < d.Mx; ++i) al[i] = (double)i / p.aphi;
+  for (int j = 0; j < d.Mz; ++j) be[j] = (double)(j <= d.Mz / 2 ? j : j - d.Mz) / p.beta;
+  CHF_TRY(hipMemcpy(c->alpha, al.data(), d.Mx * sizeof(double), hipMemcpyHostToDevice));
+  CHF_TRY(hipMemcpy(c->beta, be.data(), d.Mz * sizeof(double), hipMemcpyHostToDevice));
+
+  // SCHEB operator matrices
+  std::vector<double> t0, t1, t2, t4;
+  scheb(d.N, t0, t1, t2, t4);
+  const size_t tt = (size_t)d.Ny * d.Ny;
+  CHF_TRY(dalloc(c, &c->t0, tt));
+  CHF_TRY(dalloc(c, &c->t2, tt));
+  CHF_TRY(dalloc(c, &c->t4, tt));
+  CHF_TRY(hipMemcpy(c->t0, t0.data(), tt * sizeof(double), hipMemcpyHostToDevice));
+  CHF_TRY(hipMemcpy(c->t2, t2.data(), tt * sizeof(double), hipMemcpyHostToDevice));
+  CHF_TRY(hipMemcpy(c->t4, t4.data(), tt * sizeof(double), hipMemcpyHostToDevice));
+
+  // transform matrices: synthesis on the 2N+1 grid (folded rows p = 0..N),
+  // analysis weights (PF:1696-1700: /2N, c_0 = 2, end points weighted once),
+  // synthesis on the N+1 grid (folded rows j = 0..N/2)
+  const int N = d.N;
+  std::vector<double> ts((size_t)d.Ph * d.Ny), ta((size_t)d.Ny * d.Ph), tp((size_t)d.Pn * d.Ny);
+  for (int pp = 0; pp < d.Ph; ++pp)
+    for (int l = 0; l <= N; ++l) ts[(size_t)pp * d.Ny + l] = cospi_frac((long)pp * l, 2L * N);
+  for (int k = 0; k <= N; ++k)
+    for (int pp = 0; pp < d.Ph; ++pp) {
+      const double ck = (k == 0) ? 2.0 : 1.0;
+      const double wp = (pp == 0) ? 1.0 : 2.0;
+      ta[(size_t)k * d.Ph + pp] = wp * cospi_frac((long)k * pp, 2L * N) / (2.0 * N * ck);
+    }
+  for (int j = 0; j < d.Pn; ++j)
+    for (int l = 0; l <= N; ++l) tp[(size_t)j * d.Ny + l] = cospi_frac((long)j * l, (long)N);
+  CHF_TRY(dalloc(c, &c->tsyn, ts.size()));
+  CHF_TRY(dalloc(c, &c->tana, ta.size()));
+  CHF_TRY(dalloc(c, &c->tpts, tp.size()));
+  CHF_TRY(hipMemcpy(c->tsyn, ts.data(), ts.size() * sizeof(double), hipMemcpyHostToDevice));
+  CHF_TRY(hipMemcpy(c->tana, ta.data(), ta.size() * sizeof(double), hipMemcpyHostToDevice));
+  CHF_TRY(hipMemcpy(c->tpts, tp.data(), tp.size() * sizeof(double), hipMemcpyHostToDevice));
+
+  // FFT twiddles e^{-2 pi i m/L}
+  auto twiddles = [](int L) {
+    std::vector<double2> t(L);
+    for (int m = 0; m < L; ++m)
+      t[m] = make_double2(cospi_frac(2L * m, L), -std::sin(2.0 * M_PI * (double)m / (double)L));
+    return t;
+  };
+  {
+    auto tx = twiddles(d.Lx), tz = twiddles(d.Lz);
+    CHF_TRY(dalloc(c, &c->twx, d.Lx));
+    CHF_TRY(dalloc(c, &c->twz, d.Lz));
+    CHF_TRY(hipMemcpy(c->twx, tx.data(), d.Lx * sizeof(double2), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->twz, tz.data(), d.Lz * sizeof(double2), hipMemcpyHostToDevice));
+  }
+
+  // zero the state
+  for (double2* q : {c->v, c->oy, c->u, c->w, c->ox, c->oz, c->hv[0], c->hv[1], c->hg[0], c->hg[1]})
+    CHF_TRY(hipMemsetAsync(q, 0, fld * sizeof(double2), c->stream));
+  for (double2* q : {c->cl1[0], c->cl1[1], c->cl3[0], c->cl3[1], c->au0, c->w0})
+    CHF_TRY(hipMemsetAsync(q, 0, d.Ny * sizeof(double2), c->stream));
+
+  // kernels that stage more than 64 KiB of LDS
+  CHF_TRY(launch_setup_inverses(*c));
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+hipEvent_t next_event(Ctx* c) {
+  if (c->pool_next == c->event_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->event_pool.push_back(e);
+  }
+  return c->event_pool[c->pool_next++];
+}
+
+// Runs one launcher; with timing on, brackets it with events on the stream.
+template <class F>
+hipError_t timed(Ctx* c, int id, F&& launch) {
+  if (!c->timing) return launch();
+  hipEvent_t a = next_event(c), b = next_event(c);
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, c->stream);
+  if (e != hipSuccess) return e;
+  e = launch();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(b, c->stream);
+  if (e != hipSuccess) return e;
+  c->marks.push_back({id, a, b});
+  return hipSuccess;
+}
+
+int nonlinear(Ctx* c) {
+  // six fields in the order the streamwise pass pairs them: (u,v) (w,ox) (oy,oz)
+  const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
+  CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return launch_synth_padded(*c, in6, c->Y, 6); }));
+  CHF_TRY(timed(c, K_ZFFT_INV, [&] { return launch_zfft_inv(*c, c->Y, c->Z, 6); }));
+  CHF_TRY(timed(c, K_XPASS, [&] { return launch_xpass(*c); }));
+  CHF_TRY(timed(c, K_ZFFT_FWD, [&] { return launch_zfft_fwd(*c, c->Z, c->Y, 3); }));
+  const double2* y3[3] = {c->Y[0], c->Y[1], c->Y[2]};
+  CHF_TRY(timed(c, K_ANALYSIS, [&] { return launch_analysis(*c, y3, c->Lc, 3); }));
+  CHF_TRY(timed(c, K_HASSEMBLE, [&] { return launch_hassemble(*c); }));
+  const int nxt = 1 - c->hcur;
+  const double2* hc[2] = {c->hvc, c->hgc};
+  double2* ho[2] = {c->hv[nxt], c->hg[nxt]};
+  CHF_TRY(timed(c, K_SYNTH_POINTS, [&] { return launch_synth_points(*c, hc, ho, 2); }));
+  CHF_TRY(timed(c, K_MEAN_POINTS, [&] { return launch_mean_points(*c); }));
+  c->hcur = nxt;
+  return 0;
+}
+
+int solve(Ctx* c) {
+  CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
+  CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
+  CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c, 0); }));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chf_create(const chf_params* prm, void** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  std::string msg;
+  if (validate(prm, msg)) {
+    g_create_err = msg;
+    return -1;
+  }
+  Ctx* c = new Ctx();
+  c->prm = *prm;
+  int rc = build(c);
+  if (rc) {
+    g_create_err = c->err;
+    free_all(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int chf_destroy(void* ctx) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return 0;
+}
+
+const char* chf_last_error(void* ctx) {
+  if (!ctx) return g_create_err.c_str();
+  return static_cast<Ctx*>(ctx)->err.c_str();
+}
+
+int chf_set_state(void* ctx, const chf_state* st) {
+  if (!ctx || !st) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  const Dims& d = c->d;
+  const size_t fb = (size_t)d.Ny * d.ncol * sizeof(double2), mb = d.Ny * sizeof(double2);
+  const int n = c->hcur, pv = 1 - c->hcur;
+  struct { double2* dst; const double* src; size_t bytes; } m[] = {
+      {c->cl1[n], st->clamb1n, mb}, {c->cl1[pv], st->clamb1nm1, mb},
+      {c->cl3[n], st->clamb3n, mb}, {c->cl3[pv], st->clamb3nm1, mb},
+      {c->au0, st->au0, mb}, {c->w0, st->w0, mb},
+      {c->v, st->vnp1, fb}, {c->oy, st->omegaynp1, fb},
+      {c->hv[n], st->rhsvn, fb}, {c->hg[n], st->rhsomegayn, fb},
+      {c->hv[pv], st->unp1, fb}, {c->hg[pv], st->wnp1, fb}};
+  for (auto& e : m)
+    if (!e.src) return fail(c, "chf_set_state: null array");
+  for (auto& e : m) CHF_TRY(hipMemcpyAsync(e.dst, e.src, e.bytes, hipMemcpyHostToDevice, c->stream));
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int chf_get_state(void* ctx, const chf_state* st) {
+  if (!ctx || !st) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  const Dims& d = c->d;
+  const size_t fb = (size_t)d.Ny * d.ncol * sizeof(double2), mb = d.Ny * sizeof(double2);
+  const int n = c->hcur, pv = 1 - c->hcur;
+  struct { const double2* src; double* dst; size_t bytes; } m[] = {
+      {c->cl1[n], st->clamb1n, mb}, {c->cl1[pv], st->clamb1nm1, mb},
+      {c->cl3[n], st->clamb3n, mb}, {c->cl3[pv], st->clamb3nm1, mb},
+      {c->au0, st->au0, mb}, {c->w0, st->w0, mb},
+      {c->v, st->vnp1, fb}, {c->oy, st->omegaynp1, fb},
+      {c->hv[n], st->rhsvn, fb}, {c->hg[n], st->rhsomegayn, fb},
+      {c->hv[pv], st->unp1, fb}, {c->hg[pv], st->wnp1, fb}};
+  for (auto& e : m)
+    if (!e.dst) return fail(c, "chf_get_state: null array");
+  for (auto& e : m) CHF_TRY(hipMemcpyAsync(e.dst, e.src, e.bytes, hipMemcpyDeviceToHost, c->stream));
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1, const double* au0,
+                    const double* w0) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!vnp1 || !omegaynp1 || !au0 || !w0) return fail(c, "chf_init_fields: null array");
+  const Dims& d = c->d;
+  const size_t fb = (size_t)d.Ny * d.ncol * sizeof(double2), mb = d.Ny * sizeof(double2);
+  CHF_TRY(hipMemcpyAsync(c->v, vnp1, fb, hipMemcpyHostToDevice, c->stream));
+  CHF_TRY(hipMemcpyAsync(c->oy, omegaynp1, fb, hipMemcpyHostToDevice, c->stream));
+  CHF_TRY(hipMemcpyAsync(c->au0, au0, mb, hipMemcpyHostToDevice, c->stream));
+  CHF_TRY(hipMemcpyAsync(c->w0, w0, mb, hipMemcpyHostToDevice, c->stream));
+  // h^n = 0 and L^n = 0 so that the rotation in getnonlinear leaves h^{-1} = 0 (PF:2316)
+  for (double2* q : {c->hv[0], c->hv[1], c->hg[0], c->hg[1]})
+    CHF_TRY(hipMemsetAsync(q, 0, fb, c->stream));
+  for (double2* q : {c->cl1[0], c->cl1[1], c->cl3[0], c->cl3[1]})
+    CHF_TRY(hipMemsetAsync(q, 0, mb, c->stream));
+  // fields with all-zero u, w, omega_x, omega_z at the Nyquist column
+  for (double2* q : {c->u, c->w, c->ox, c->oz}) CHF_TRY(hipMemsetAsync(q, 0, fb, c->stream));
+  CHF_TRY(launch_solve(*c, 1));
+  int rc = nonlinear(c);
+  if (rc) return rc;
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int chf_nonlinear(void* ctx) {
+  if (!ctx) return -1;
+  return nonlinear(static_cast<Ctx*>(ctx));
+}
+
+int chf_solve(void* ctx) {
+  if (!ctx) return -1;
+  return solve(static_cast<Ctx*>(ctx));
+}
+
+int chf_step(void* ctx, int n) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (n < 0) return fail(c, "chf_step: negative step count");
+  for (int i = 0; i < n; ++i) {
+    int rc = solve(c);
+    if (rc) return rc;
+    rc = nonlinear(c);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int chf_sync(void* ctx) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int chf_get_fields(void* ctx, double* unp1, double* wnp1, double* omegaxnp1, double* omegaznp1) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  const size_t fb = (size_t)c->d.Ny * c->d.ncol * sizeof(double2);
+  struct { const double2* src; double* dst; } m[] = {
+      {c->u, unp1}, {c->w, wnp1}, {c->ox, omegaxnp1}, {c->oz, omegaznp1}};
+  for (auto& e : m)
+    if (e.dst) CHF_TRY(hipMemcpyAsync(e.dst, e.src, fb, hipMemcpyDeviceToHost, c->stream));
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int chf_dims(void* ctx, int* mx, int* mz, int* ny) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (mx) *mx = c->d.Mx;
+  if (mz) *mz = c->d.Mz;
+  if (ny) *ny = c->d.Ny;
+  return 0;
+}
+
+long long chf_device_bytes(void* ctx) {
+  if (!ctx) return -1;
+  return static_cast<Ctx*>(ctx)->bytes;
+}
+
+int chf_set_timing(void* ctx, int on) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  c->timing = on != 0;
+  c->marks.clear();
+  c->pool_next = 0;
+  return 0;
+}
+
+int chf_kernel_count(void) { return K_COUNT; }
+
+const char* chf_kernel_name(int id) {
+  if (id < 0 || id >= K_COUNT) return "";
+  return kKernelNames[id];
+}
+
+int chf_timing(void* ctx, int id, double* total_ms, int* launches) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (id < 0 || id >= K_COUNT) return fail(c, "chf_timing: kernel id out of range");
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  int n = 0;
+  for (const auto& m : c->marks) {
+    if (m.kernel != id) continue;
+    float ms = 0.f;
+    CHF_TRY(hipEventElapsedTime(&ms, m.a, m.b));
+    tot += ms;
+    ++n;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = n;
+  return 0;
+}
+
+}  // extern "C"

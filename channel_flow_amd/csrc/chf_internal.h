@@ -1,0 +1,117 @@
+// Internal declarations shared by the HIP translation units of libchf.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/chf.h"
+
+namespace chf {
+
+// Sizes derived from chf_params (reference symbols in brackets).
+struct Dims {
+  int Mx;     // streamwise modes kept, nnx/2 [nhx+1]
+  int Mz;     // spanwise modes incl. Nyquist, nny [ny+1]
+  int N;      // Chebyshev degree [nz]
+  int Ny;     // N+1 coefficients / points
+  int Lx;     // padded streamwise points 3 nnx/2 [plan9/10 length]
+  int Lz;     // padded spanwise points 3 nny/2   [plan8/11 length]
+  int Py;     // padded wall-normal points 2N+1   [plan13 grid, PF:1647-1667]
+  int Ph;     // Py points folded by parity: N+1
+  int ne;     // even Chebyshev degrees 0,2,..: N/2+1
+  int no;     // odd degrees 1,3,..: (N+1)/2
+  int Pn;     // N+1-point grid folded by parity: N/2+1
+  long ncol;  // complex columns per plane: Mz*Mx
+  int ngroups;  // (kx,|kz|) operator groups: Mx*(Mz/2)
+  long inv_stride;  // doubles per group per equation: ne^2 + no^2
+};
+
+// Device-side view of the wall-normal matrices (all point-major, row = point).
+struct Mats {
+  const double* tsyn;   // [Ph][Ny]   T_l(y_p), y_p = cos(pi p/2N), p = 0..N
+  const double* tana;   // [Ny][Ph]   analysis weights folded onto p = 0..N
+  const double* tpts;   // [Pn][Ny]   T_l(y_j), y_j = cos(pi j/N), j = 0..N/2
+  const double* t0;     // [Ny][Ny]   SCHEB T0 (operators)
+  const double* t2;     // [Ny][Ny]   SCHEB T2
+  const double* t4;     // [Ny][Ny]   SCHEB T4
+};
+
+struct Ctx {
+  chf_params prm{};
+  Dims d{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // wavenumbers
+  double* alpha = nullptr;  // [Mx]
+  double* beta = nullptr;   // [Mz]
+  // state (complex, [Ny][Mz][Mx] unless noted)
+  double2 *v = nullptr, *oy = nullptr;              // coefficients
+  double2 *u = nullptr, *w = nullptr, *ox = nullptr, *oz = nullptr;  // derived
+  double2* hv[2] = {nullptr, nullptr};  // h_v point values; [hcur] = h^n
+  double2* hg[2] = {nullptr, nullptr};  // h_g point values
+  double2* cl1[2] = {nullptr, nullptr};  // mean L_x(0,0) point values [Ny]
+  double2* cl3[2] = {nullptr, nullptr};  // mean L_z(0,0) point values [Ny]
+  double2 *au0 = nullptr, *w0 = nullptr;  // mean U, W coefficients [Ny]
+  int hcur = 0;
+  // nonlinear work
+  double2* Y[6] = {};     // [Py][Mz][Mx]   wall-normal points, Fourier modes
+  double2* Z[6] = {};     // [Py][Lz][Mx]   wall-normal + span points, x modes
+  double2* Lc[3] = {};    // [Ny][Mz][Mx]   Lamb vector coefficients
+  double2 *hvc = nullptr, *hgc = nullptr;  // h coefficients
+  // solve work
+  double2 *rvE = nullptr, *rvO = nullptr, *roE = nullptr, *roO = nullptr;
+  // operators
+  double* host_mats = nullptr;
+  double *tsyn = nullptr, *tana = nullptr, *tpts = nullptr, *t0 = nullptr, *t2 = nullptr,
+         *t4 = nullptr;
+  double* invV = nullptr;  // [ngroups][ne^2 + no^2] transposed inverses (v)
+  double* invW = nullptr;  // [ngroups][ne^2 + no^2] (omega_y; group 0 = mean mode)
+  double2* twx = nullptr;  // exp(-2 pi i m/Lx), m < Lx
+  double2* twz = nullptr;  // exp(-2 pi i m/Lz), m < Lz
+  long long bytes = 0;
+  std::string err;
+  // optional per-launch HIP-event timing on the context stream
+  bool timing = false;
+  struct Mark {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> event_pool;
+  size_t pool_next = 0;
+
+  Mats mats() const { return Mats{tsyn, tana, tpts, t0, t2, t4}; }
+};
+
+// FFT plan: Stockham stages with radices 4, 2, 3.
+struct FftPlan {
+  int L;
+  int nst;
+  int rad[16];
+};
+FftPlan make_plan(int L);
+bool fft_length_ok(int L);
+
+// Kernel classes for per-launch timing.
+enum KernelId {
+  K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
+  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_COUNT
+};
+extern const char* const kKernelNames[K_COUNT];
+
+// ---- kernel launchers (each returns hipGetLastError()) ----
+hipError_t launch_zfft_inv(Ctx& c, double2* const* in, double2* const* out, int nf);
+hipError_t launch_xpass(Ctx& c);
+hipError_t launch_zfft_fwd(Ctx& c, double2* const* in, double2* const* out, int nf);
+hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const* out, int nf);
+hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
+hipError_t launch_hassemble(Ctx& c);
+hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf);
+hipError_t launch_mean_points(Ctx& c);
+hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
+hipError_t launch_solve(Ctx& c, int mode);  // mode 0 = solve, 1 = kinematics only
+hipError_t launch_setup_inverses(Ctx& c);
+
+}  // namespace chf

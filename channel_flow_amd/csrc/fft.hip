@@ -1,0 +1,325 @@
+// 3/2-padded Fourier transforms of the nonlinear term (ftrthreetotwo,
+// PF:1024-1185, with expand PF:721-765 and getall PF:1350-1362), written as
+// LDS-resident Stockham FFTs over batches of lines.
+//
+//   zfft_inv : span modes -> span points, padded Mz -> Lz = 3 nny/2, Nyquist
+//              zeroed (plan8, PF:1058-1082)
+//   xpass    : for each (y point, span point) line: three packed c2r
+//              streamwise FFTs of the six fields (plan9, Hermitian fill and
+//              Re part, PF:1097-1119), the Lamb vector
+//              L = omega x u - re_p x^ (PF:1438-1473, re_p at PF:1787), two
+//              packed r2c FFTs truncated to Mx modes (plan10, PF:1130-1143)
+//              and the 1/(Lx Lz) scaling (PF:1176-1177)
+//   zfft_fwd : span points -> span modes, truncated keeping slot Mz/2
+//              (plan11 and expand(-1), PF:1158-1179)
+#include "chf_internal.h"
+
+namespace chf {
+
+bool fft_length_ok(int L) {
+  if (L < 2) return false;
+  while (L % 2 == 0) L /= 2;
+  while (L % 3 == 0) L /= 3;
+  return L == 1;
+}
+
+FftPlan make_plan(int L) {
+  FftPlan p{};
+  p.L = L;
+  int m = L;
+  while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
+  return p;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Batched Stockham FFT of nl lines of length L held in LDS (line stride L).
+// sign = -1 forward (e^{-i}), +1 backward; unnormalised.  tw[m] = e^{-2 pi i m/L}.
+// Ping-pongs between a and b; returns the buffer holding the result.
+__device__ double2* fft_lds(double2* a, double2* b, const FftPlan& pl, int nl, int sign,
+                            const double2* __restrict__ tw) {
+  const int L = pl.L;
+  int Ns = 1;
+  double2* src = a;
+  double2* dst = b;
+  const double s3 = 0.86602540378443864676 * sign;
+  for (int st = 0; st < pl.nst; ++st) {
+    const int R = pl.rad[st];
+    const int NB = L / R;
+    const int tstep = L / (Ns * R);
+    for (int t = threadIdx.x; t < nl * NB; t += blockDim.x) {
+      const int line = t / NB;
+      const int j = t - line * NB;
+      const double2* s = src + line * L;
+      double2* d = dst + line * L;
+      const int k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+      double2 v0 = s[j], v1 = s[j + NB];
+      if (R == 2) {
+        if (k) {
+          double2 w = tw[k * tstep];
+          if (sign > 0) w.y = -w.y;
+          v1 = cmul(v1, w);
+        }
+        d[base] = make_double2(v0.x + v1.x, v0.y + v1.y);
+        d[base + Ns] = make_double2(v0.x - v1.x, v0.y - v1.y);
+      } else if (R == 4) {
+        double2 v2 = s[j + 2 * NB], v3 = s[j + 3 * NB];
+        if (k) {
+          int i1 = k * tstep, i2 = 2 * k * tstep, i3 = 3 * k * tstep;
+          double2 w1 = tw[i1], w2 = tw[i2], w3 = tw[i3];
+          if (sign > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+          v1 = cmul(v1, w1);
+          v2 = cmul(v2, w2);
+          v3 = cmul(v3, w3);
+        }
+        double2 a0 = make_double2(v0.x + v2.x, v0.y + v2.y);
+        double2 a1 = make_double2(v0.x - v2.x, v0.y - v2.y);
+        double2 a2 = make_double2(v1.x + v3.x, v1.y + v3.y);
+        double2 a3 = make_double2(v1.x - v3.x, v1.y - v3.y);
+        // (sign i) * a3
+        double2 ia3 = make_double2(-sign * a3.y, sign * a3.x);
+        d[base] = make_double2(a0.x + a2.x, a0.y + a2.y);
+        d[base + Ns] = make_double2(a1.x + ia3.x, a1.y + ia3.y);
+        d[base + 2 * Ns] = make_double2(a0.x - a2.x, a0.y - a2.y);
+        d[base + 3 * Ns] = make_double2(a1.x - ia3.x, a1.y - ia3.y);
+      } else {  // R == 3
+        double2 v2 = s[j + 2 * NB];
+        if (k) {
+          int i1 = k * tstep, i2 = 2 * k * tstep;
+          double2 w1 = tw[i1], w2 = tw[i2];
+          if (sign > 0) { w1.y = -w1.y; w2.y = -w2.y; }
+          v1 = cmul(v1, w1);
+          v2 = cmul(v2, w2);
+        }
+        double2 sum = make_double2(v1.x + v2.x, v1.y + v2.y);
+        double2 dif = make_double2(v1.x - v2.x, v1.y - v2.y);
+        double2 m = make_double2(v0.x - 0.5 * sum.x, v0.y - 0.5 * sum.y);
+        // (sign i sqrt3/2) * dif
+        double2 r = make_double2(-s3 * dif.y, s3 * dif.x);
+        d[base] = make_double2(v0.x + sum.x, v0.y + sum.y);
+        d[base + Ns] = make_double2(m.x + r.x, m.y + r.y);
+        d[base + 2 * Ns] = make_double2(m.x - r.x, m.y - r.y);
+      }
+    }
+    __syncthreads();
+    double2* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+// ----------------------------------------------------------------- span
+
+struct ZArgs {
+  const double2* in[6];
+  double2* out[6];
+  int Mx, Mz, Lz, TX;
+  long in_plane;   // complex elements per wall-normal plane of the input
+  long out_plane;  // ... of the output
+  FftPlan pl;
+  const double2* tw;
+};
+
+// grid: (Mx/TX, Py, nfields); one block transforms TX span lines of one plane.
+__global__ __launch_bounds__(256) void k_zfft_inv(ZArgs a) {
+  extern __shared__ double2 sm[];
+  const int TX = a.TX, Lz = a.Lz, Mz = a.Mz, h = Mz / 2;
+  double2* b0 = sm;
+  double2* b1 = sm + TX * Lz;
+  const int kx0 = blockIdx.x * TX;
+  const int p = blockIdx.y;
+  const double2* in = a.in[blockIdx.z] + (long)p * a.in_plane;
+  double2* out = a.out[blockIdx.z] + (long)p * a.out_plane;
+  for (int idx = threadIdx.x; idx < TX * Lz; idx += blockDim.x) {
+    const int m = idx / TX, tx = idx - m * TX;
+    int kz = -1;
+    if (m < h) kz = m;                       // expand(+1): 0..Mz/2-1, Nyquist zeroed
+    else if (m > Lz - h) kz = m - Lz + Mz;   // negative modes
+    double2 val = make_double2(0.0, 0.0);
+    if (kz >= 0) val = in[(long)kz * a.Mx + kx0 + tx];
+    b0[tx * Lz + m] = val;
+  }
+  __syncthreads();
+  double2* r = fft_lds(b0, b1, a.pl, TX, +1, a.tw);
+  for (int idx = threadIdx.x; idx < TX * Lz; idx += blockDim.x) {
+    const int m = idx / TX, tx = idx - m * TX;
+    out[(long)m * a.Mx + kx0 + tx] = r[tx * Lz + m];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_zfft_fwd(ZArgs a) {
+  extern __shared__ double2 sm[];
+  const int TX = a.TX, Lz = a.Lz, Mz = a.Mz, h = Mz / 2;
+  double2* b0 = sm;
+  double2* b1 = sm + TX * Lz;
+  const int kx0 = blockIdx.x * TX;
+  const int p = blockIdx.y;
+  const double2* in = a.in[blockIdx.z] + (long)p * a.in_plane;
+  double2* out = a.out[blockIdx.z] + (long)p * a.out_plane;
+  for (int idx = threadIdx.x; idx < TX * Lz; idx += blockDim.x) {
+    const int m = idx / TX, tx = idx - m * TX;
+    b0[tx * Lz + m] = in[(long)m * a.Mx + kx0 + tx];
+  }
+  __syncthreads();
+  double2* r = fft_lds(b0, b1, a.pl, TX, -1, a.tw);
+  for (int idx = threadIdx.x; idx < TX * Mz; idx += blockDim.x) {
+    const int kz = idx / TX, tx = idx - kz * TX;
+    const int m = (kz <= h) ? kz : kz - Mz + Lz;  // expand(-1) keeps slot Mz/2
+    out[(long)kz * a.Mx + kx0 + tx] = r[tx * Lz + m];
+  }
+}
+
+static int pick_tx(int Mx, int Lz) {
+  int tx = 1;
+  for (int c : {16, 8, 4, 2}) {
+    if (Mx % c == 0 && c * Lz <= 2048) { tx = c; break; }
+  }
+  return tx;
+}
+
+static hipError_t launch_z(Ctx& c, double2* const* in, double2* const* out, int nf, bool inv) {
+  const Dims& d = c.d;
+  ZArgs a{};
+  for (int f = 0; f < nf; ++f) {
+    a.in[f] = in[f];
+    a.out[f] = out[f];
+  }
+  a.Mx = d.Mx;
+  a.Mz = d.Mz;
+  a.Lz = d.Lz;
+  a.TX = pick_tx(d.Mx, d.Lz);
+  a.in_plane = (long)(inv ? d.Mz : d.Lz) * d.Mx;
+  a.out_plane = (long)(inv ? d.Lz : d.Mz) * d.Mx;
+  a.pl = make_plan(d.Lz);
+  a.tw = c.twz;
+  dim3 grid(d.Mx / a.TX, d.Py, nf);
+  size_t shm = 2ull * a.TX * d.Lz * sizeof(double2);
+  if (inv)
+    hipLaunchKernelGGL(k_zfft_inv, grid, dim3(256), shm, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_zfft_fwd, grid, dim3(256), shm, c.stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_zfft_inv(Ctx& c, double2* const* in, double2* const* out, int nf) {
+  return launch_z(c, in, out, nf, true);
+}
+hipError_t launch_zfft_fwd(Ctx& c, double2* const* in, double2* const* out, int nf) {
+  return launch_z(c, in, out, nf, false);
+}
+
+// ------------------------------------------------------------- streamwise
+
+struct XArgs {
+  const double2* in[6];  // u, v, w, omega_x, omega_y, omega_z  [Py][Lz][Mx]
+  double2* out[3];       // L_x, L_y, L_z                        [Py][Lz][Mx]
+  int Mx, Lx, LPB;
+  long nlines;           // Py * Lz
+  double re_p, scale;
+  FftPlan pl;
+  const double2* tw;
+};
+
+// One block handles LPB (y point, span point) lines: 3 packed inverse FFTs,
+// products, 2 packed forward FFTs.
+__global__ __launch_bounds__(256) void k_xpass(XArgs a) {
+  extern __shared__ double2 sm[];
+  const int Lx = a.Lx, Mx = a.Mx, LPB = a.LPB;
+  const int nl = 3 * LPB;
+  double2* b0 = sm;
+  double2* b1 = sm + nl * Lx;
+  const long line0 = (long)blockIdx.x * LPB;
+  // build the packed padded spectra: pair q holds fields (2q, 2q+1)
+  for (int idx = threadIdx.x; idx < nl * Lx; idx += blockDim.x) {
+    const int ln = idx / Lx, m = idx - ln * Lx;
+    const int lb = ln / 3, q = ln - lb * 3;
+    const long line = line0 + lb;
+    double2 val = make_double2(0.0, 0.0);
+    if (line < a.nlines) {
+      const double2* f1 = a.in[2 * q] + line * Mx;
+      const double2* f2 = a.in[2 * q + 1] + line * Mx;
+      if (m == 0) {
+        // Hermitian fill keeps Re of the kx=0 entry (Re of the backward FFT, PF:1116-1118)
+        val = make_double2(f1[0].x, f2[0].x);
+      } else if (m < Mx) {
+        double2 A = f1[m], B = f2[m];
+        val = make_double2(A.x - B.y, A.y + B.x);          // F1 + i F2
+      } else if (m > Lx - Mx) {
+        const int mm = Lx - m;
+        double2 A = f1[mm], B = f2[mm];
+        val = make_double2(A.x + B.y, B.x - A.y);          // conj(F1) + i conj(F2)
+      }
+    }
+    b0[idx] = val;
+  }
+  __syncthreads();
+  double2* r = fft_lds(b0, b1, a.pl, nl, +1, a.tw);
+  double2* o = (r == b0) ? b1 : b0;
+  // products at the Lx physical points; pack (L_x + i L_y) and (L_z + 0 i)
+  for (int idx = threadIdx.x; idx < LPB * Lx; idx += blockDim.x) {
+    const int lb = idx / Lx, x = idx - lb * Lx;
+    const double2 g0 = r[(3 * lb + 0) * Lx + x];  // u + i v
+    const double2 g1 = r[(3 * lb + 1) * Lx + x];  // w + i omega_x
+    const double2 g2 = r[(3 * lb + 2) * Lx + x];  // omega_y + i omega_z
+    const double uu = g0.x, vv = g0.y, ww = g1.x, oxx = g1.y, oyy = g2.x, ozz = g2.y;
+    const double lx = (oyy * ww - a.re_p) - ozz * vv;
+    const double ly = ozz * uu - oxx * ww;
+    const double lz = oxx * vv - oyy * uu;
+    o[(2 * lb + 0) * Lx + x] = make_double2(lx, ly);
+    o[(2 * lb + 1) * Lx + x] = make_double2(lz, 0.0);
+  }
+  __syncthreads();
+  double2* other = (o == b0) ? b1 : b0;
+  double2* f = fft_lds(o, other, a.pl, 2 * LPB, -1, a.tw);
+  for (int idx = threadIdx.x; idx < LPB * Mx; idx += blockDim.x) {
+    const int lb = idx / Mx, m = idx - lb * Mx;
+    const long line = line0 + lb;
+    if (line >= a.nlines) continue;
+    const double2 G = f[(2 * lb) * Lx + m];
+    const double2 Gc = f[(2 * lb) * Lx + (m == 0 ? 0 : Lx - m)];
+    // F1 = (G(m) + conj(G(-m)))/2 ; F2 = (G(m) - conj(G(-m)))/(2i)
+    const double2 F1 = make_double2(0.5 * (G.x + Gc.x), 0.5 * (G.y - Gc.y));
+    const double2 F2 = make_double2(0.5 * (G.y + Gc.y), -0.5 * (G.x - Gc.x));
+    const double2 F3 = f[(2 * lb + 1) * Lx + m];
+    const double s = a.scale;
+    a.out[0][line * Mx + m] = make_double2(F1.x * s, F1.y * s);
+    a.out[1][line * Mx + m] = make_double2(F2.x * s, F2.y * s);
+    a.out[2][line * Mx + m] = make_double2(F3.x * s, F3.y * s);
+  }
+}
+
+hipError_t launch_xpass(Ctx& c) {
+  const Dims& d = c.d;
+  XArgs a{};
+  for (int f = 0; f < 6; ++f) a.in[f] = c.Z[f];
+  // outputs overwrite Z[0..2] line by line: each block reads its own lines fully
+  // before writing them, and no block touches another block's lines
+  for (int f = 0; f < 3; ++f) a.out[f] = c.Z[f];
+  a.Mx = d.Mx;
+  a.Lx = d.Lx;
+  int lpb = 2048 / (3 * d.Lx);
+  if (lpb < 1) lpb = 1;
+  a.LPB = lpb;
+  a.nlines = (long)d.Py * d.Lz;
+  a.re_p = c.prm.re_p;
+  a.scale = 1.0 / ((double)d.Lx * (double)d.Lz);
+  a.pl = make_plan(d.Lx);
+  a.tw = c.twx;
+  dim3 grid((unsigned)((a.nlines + lpb - 1) / lpb));
+  size_t shm = 2ull * 3 * lpb * d.Lx * sizeof(double2);
+  if (shm > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_xpass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_xpass, grid, dim3(256), shm, c.stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace chf

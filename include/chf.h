@@ -1,0 +1,105 @@
+/*
+ * chf.h - C ABI of the MI355X per-timestep core of the Fourier-Fourier-Chebyshev
+ * channel-flow DNS (v - omega_y formulation, Crank-Nicolson / Adams-Bashforth-2).
+ *
+ * The reference solver (poiseuilleflow.f90 = PF, paraandcons.f90 = PC) has no
+ * plugin or operator API: its time loop (PF:220-532) calls Fortran external
+ * subroutines that share state through module arrays (PC:28-38).  Each entry
+ * point below replaces one of those call sites; the comment names it.
+ *
+ * Conventions
+ *   - All arrays are host pointers in the reference memory layout:
+ *       spectral/point field  real*8 (0:nx,0:ny,0:nz), Re/Im interleaved along
+ *                             dim 1 == complex128[Ny][Mz][Mx], C order,
+ *                             Mx = nnx/2, Mz = nny, Ny = nnz+1;
+ *       mean-mode array       real*8 (2,0:nz) == complex128[Ny].
+ *   - Return value: 0 ok; < 0 invalid argument (message in chf_last_error);
+ *     > 0 a HIP runtime error code.
+ *   - One context = one channel on the current HIP device and one stream.
+ *     Calls on one context must come from one host thread.
+ */
+#ifndef CHF_H
+#define CHF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compile-time parameters of PC:4-26 that the timestep core reads (P = 1). */
+typedef struct chf_params {
+  int nnx;        /* streamwise points (PC:5); nnx % 4 == 0                    */
+  int nny;        /* spanwise points (PC:6); nny % 4 == 0                      */
+  int nnz;        /* Chebyshev degree N (PC:7); N >= 4                         */
+  double aphi;    /* Lx = 2 pi aphi (PC:21)                                    */
+  double beta;    /* Lz = 2 pi beta (PC:21)                                    */
+  double dt;      /* time step (PC:23)                                         */
+  double re;      /* Reynolds number (PC:24)                                   */
+  double re_p;    /* mean pressure gradient (PC:25)                            */
+  int precision;  /* 0 = FP64 throughout; 1 = mixed (FP32 transforms/products, */
+                  /*     FP64 solves and state)                                */
+} chf_params;
+
+/* Pre-solve state = the restart record of PF:2476-2479 (output) and
+ * PF:1913-1917 (read back).  Field arrays are complex128[Ny][Mz][Mx], mean
+ * arrays complex128[Ny].  h-arrays hold wall-normal point values y_j =
+ * cos(pi j/N); v, omega_y, au0, w0 hold Chebyshev coefficients. */
+typedef struct chf_state {
+  double *clamb1n, *clamb1nm1, *clamb3n, *clamb3nm1, *au0, *w0;  /* record 1 */
+  double *vnp1, *omegaynp1, *rhsvn, *rhsomegayn, *unp1, *wnp1;   /* record 2 */
+} chf_state;
+
+/* Builds the per-(kx,|kz|) operator inverses on the GPU (replaces SCHEB PF:156
+ * and the get_coef_v/get_coef_omegay + zgetrf/zgetri set-up loop, PF:181-204). */
+int chf_create(const chf_params* prm, void** ctx);
+int chf_destroy(void* ctx);
+const char* chf_last_error(void* ctx);
+
+/* Host -> device copy of a pre-solve state (replaces initial(isign=1),
+ * PF:1861-1924). */
+int chf_set_state(void* ctx, const chf_state* st);
+/* Device -> host copy of the pre-solve state (replaces output, PF:2460-2504). */
+int chf_get_state(void* ctx, const chf_state* st);
+
+/* Cold start (replaces initial(isign=-1) after input, PF:1932-1946): from v,
+ * omega_y (coefficients, complex128[Ny][Mz][Mx]) and the mean profiles U, W
+ * (complex128[Ny]) derive u, w, omega_x, omega_z (getvelocityandvorticity,
+ * PF:2347-2407), set h^{-1} = 0 and run one getnonlinear. */
+int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1,
+                    const double* au0, const double* w0);
+
+/* getnonlinear (PF:1380-1612): six dealiased products -> Lamb vector ->
+ * h_v, h_g, AB2 history rotation, point values. */
+int chf_nonlinear(void* ctx);
+/* Per-mode v / omega_y solves, kinematics and the mean mode (PF:242-474). */
+int chf_solve(void* ctx);
+/* n x [chf_solve ; chf_nonlinear] from a pre-solve state (PF:220-532). */
+int chf_step(void* ctx, int n);
+/* Waits for all work queued on the context's stream. */
+int chf_sync(void* ctx);
+
+/* Post-solve velocity and vorticity (unp1, wnp1, omegaxnp1, omegaznp1 as the
+ * reference holds them after its solve loop, PF:387-472), any pointer may be
+ * NULL.  Valid after chf_solve or chf_init_fields. */
+int chf_get_fields(void* ctx, double* unp1, double* wnp1, double* omegaxnp1,
+                   double* omegaznp1);
+
+/* Sizes derived from the parameters (for callers that allocate buffers). */
+int chf_dims(void* ctx, int* mx, int* mz, int* ny);
+
+/* Device bytes held by the context. */
+long long chf_device_bytes(void* ctx);
+
+/* Instrumentation (no counterpart in the reference, whose only timer is the
+ * mpi_wtime bracket of the solve loop, PF:217-229, 481-486).  With timing on,
+ * every kernel launch of chf_solve/chf_nonlinear/chf_step is bracketed by HIP
+ * events on the context stream; chf_timing sums the elapsed time of kernel
+ * class `id` (0 <= id < chf_kernel_count()) since the last chf_set_timing. */
+int chf_set_timing(void* ctx, int on);
+int chf_kernel_count(void);
+const char* chf_kernel_name(int id);
+int chf_timing(void* ctx, int id, double* total_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHF_H */

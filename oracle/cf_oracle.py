@@ -467,36 +467,57 @@ class Stepper:
             self._inv_cache[kmn] = hit
         return hit
 
-    def solve(self, st: dict) -> None:
+    def solve(self, st: dict, chunk: int = 256) -> None:
+        """Per-mode solve loop (PF:242-394) batched over modes.
+
+        Each mode's arithmetic is the reference's: rhs = explicit operator
+        times v^n (and omega_y^n) plus the AB2 forcing at the collocation
+        points (PF:252-257, 3396-3423; PF:335-340, 3472-3487), x = A^-1 rhs
+        (a_multiply_x PF:4123-4139), then the kinematics of PF:2357-2403.
+        """
         p, n, dt = self.p, self.p.N, self.p.dt
         alpha, beta = wavenumbers(p)
         v, oy = st["vnp1"], st["omegaynp1"]
         u, w = st["unp1"], st["wnp1"]
         ox, oz = st["omegaxnp1"], st["omegaznp1"]
         hv, hg = st["rhsvn"], st["rhsomegayn"]
-        for (i, j, kmn) in self.solve_modes:
-            av, ae, bv, be = self._inverses(kmn)
+        modes = self.solve_modes
+        for c0 in range(0, len(modes), chunk):
+            blk = modes[c0:c0 + chunk]
+            ii = np.array([m[0] for m in blk])
+            jj = np.array([m[1] for m in blk])
+            kk = np.array([m[2] for m in blk])
+            mats = [self._inverses(k) for k in kk]
+            av = np.stack([m[0] for m in mats])
+            ae = np.stack([m[1] for m in mats])
+            bv = np.stack([m[2] for m in mats])
+            be = np.stack([m[3] for m in mats])
             # v equation (PF:252-331)
-            f = 3.0 * dt / 2.0 * hv[:, j, i] - dt / 2.0 * u[:, j, i]
-            rhs = np.zeros(n + 1, dtype=np.complex128)
-            rhs[4:] = bv @ v[:, j, i] + f[2:n - 1]
-            vn = av @ rhs
-            v[:, j, i] = vn
+            vm = v[:, jj, ii].T                                       # (nc, Ny)
+            f = (3.0 * dt / 2.0 * hv[:, jj, ii] - dt / 2.0 * u[:, jj, ii]).T
+            rhs = np.zeros((len(blk), n + 1), dtype=np.complex128)
+            rhs[:, 4:] = np.einsum("mpl,ml->mp", bv, vm) + f[:, 2:n - 1]
+            vn = np.einsum("mkl,ml->mk", av, rhs)
             # omega_y equation (PF:335-384)
-            f = 3.0 * dt / 2.0 * hg[:, j, i] - dt / 2.0 * w[:, j, i]
-            rhs = np.zeros(n + 1, dtype=np.complex128)
-            rhs[2:] = be @ oy[:, j, i] + f[1:n]
-            on = ae @ rhs
-            oy[:, j, i] = on
+            om = oy[:, jj, ii].T
+            f = (3.0 * dt / 2.0 * hg[:, jj, ii] - dt / 2.0 * w[:, jj, ii]).T
+            rhs = np.zeros((len(blk), n + 1), dtype=np.complex128)
+            rhs[:, 2:] = np.einsum("mpl,ml->mp", be, om) + f[:, 1:n]
+            on = np.einsum("mkl,ml->mk", ae, rhs)
             # kinematics (getvelocityandvorticity PF:2357-2403)
-            a_, b_ = alpha[i], beta[j]
-            dv = dz(vn)
-            uu = (1j * a_ * dv - 1j * b_ * on) / kmn
-            ww = (1j * b_ * dv + 1j * a_ * on) / kmn
-            u[:, j, i] = uu
-            w[:, j, i] = ww
-            ox[:, j, i] = dz(ww) - 1j * b_ * vn
-            oz[:, j, i] = -dz(uu) + 1j * a_ * vn
+            a_ = alpha[ii][None, :]
+            b_ = beta[jj][None, :]
+            k2 = kk[None, :]
+            vt, ot = vn.T, on.T                                         # (Ny, nc)
+            dv = dz(vt)
+            uu = (1j * a_ * dv - 1j * b_ * ot) / k2
+            ww = (1j * b_ * dv + 1j * a_ * ot) / k2
+            v[:, jj, ii] = vt
+            oy[:, jj, ii] = ot
+            u[:, jj, ii] = uu
+            w[:, jj, ii] = ww
+            ox[:, jj, ii] = dz(ww) - 1j * b_ * vt
+            oz[:, jj, ii] = -dz(uu) + 1j * a_ * vt
         self.solve_mean(st)
 
     def solve_mean(self, st: dict) -> None:
@@ -523,21 +544,21 @@ def kinematics_all(st: dict, p: Params) -> None:
     """getvelocityandvorticity for every k^2 != 0 mode (PF:2288-2295 in input),
     plus the mean-mode velocity/vorticity from au0, w0 (PF:423-472)."""
     alpha, beta = wavenumbers(p)
-    v, oy = st["vnp1"], st["omegaynp1"]
-    for i in range(p.Mx):
-        for j in range(p.Mz):
-            kmn = alpha[i] ** 2 + beta[j] ** 2
-            if kmn == 0.0 or j == p.Mz // 2:
-                continue
-            a_, b_ = alpha[i], beta[j]
-            vn, on = v[:, j, i], oy[:, j, i]
-            dv = dz(vn)
-            uu = (1j * a_ * dv - 1j * b_ * on) / kmn
-            ww = (1j * b_ * dv + 1j * a_ * on) / kmn
-            st["unp1"][:, j, i] = uu
-            st["wnp1"][:, j, i] = ww
-            st["omegaxnp1"][:, j, i] = dz(ww) - 1j * b_ * vn
-            st["omegaznp1"][:, j, i] = -dz(uu) + 1j * a_ * vn
+    k2 = alpha[None, None, :] ** 2 + beta[None, :, None] ** 2
+    mask = np.ones((1, p.Mz, p.Mx), dtype=bool)
+    mask[0, p.Mz // 2, :] = False          # Nyquist column (PF:247)
+    mask[0, 0, 0] = False                  # k^2 = 0 (PF:2365)
+    k2s = np.where(mask, k2, 1.0)
+    a_ = alpha[None, None, :]
+    b_ = beta[None, :, None]
+    vn, on = st["vnp1"], st["omegaynp1"]
+    dv = dz(vn)
+    uu = (1j * a_ * dv - 1j * b_ * on) / k2s
+    ww = (1j * b_ * dv + 1j * a_ * on) / k2s
+    ox = dz(ww) - 1j * b_ * vn
+    oz = -dz(uu) + 1j * a_ * vn
+    for name, val in (("unp1", uu), ("wnp1", ww), ("omegaxnp1", ox), ("omegaznp1", oz)):
+        st[name] = np.where(mask, val, st[name])
     st["unp1"][:, 0, 0] = st["au0"]
     st["wnp1"][:, 0, 0] = st["w0"]
     st["omegaznp1"][:, 0, 0] = -dz(st["au0"])

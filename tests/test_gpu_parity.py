@@ -1,0 +1,153 @@
+"""Parity of the HIP timestep (through the C ABI) with the numpy oracle.
+
+Floating-point tolerance: the oracle is the reference's algorithm in FP64
+with explicit per-mode inverses; the GPU path is FP64 with parity-split
+inverses, FFT-free wall-normal contractions and different summation orders.
+Single operations agree to ~1e-13 relative; multi-step trajectories are
+held to 1e-9 relative per array (the field-error budget of the mixed form).
+"""
+import numpy as np
+import pytest
+
+from channel_flow_amd import ic
+from oracle import cf_oracle as O
+
+from .helpers import SMALL, STATE_KEYS, ic_of, params_of, rel_err, state_errors
+
+pytestmark = pytest.mark.gpu
+
+ONE_OP_TOL = 1e-11
+TRAJ_TOL = 1e-9
+
+
+def make_channel(cfg, precision="fp64"):
+    from channel_flow_amd.channel import Channel
+    return Channel(cfg["nnx"], cfg["nny"], cfg["nnz"], cfg["aphi"], cfg["beta"], cfg["dt"],
+                   cfg["re"], cfg["re_p"], precision=precision)
+
+
+def report(tag, errs):
+    print(tag, " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+
+
+@pytest.fixture(scope="module")
+def small_case(require_gpu):
+    cfg = SMALL
+    p = params_of(cfg)
+    v, oy, au0, w0 = ic_of(cfg)
+    ref = O.initial_presolve_state(v, oy, au0, w0, p)
+    return cfg, p, (v, oy, au0, w0), ref
+
+
+def test_state_roundtrip(small_case):
+    cfg, p, _, ref = small_case
+    with make_channel(cfg) as ch:
+        ch.set_state(ref)
+        got = ch.get_state()
+    for k in STATE_KEYS:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+
+
+def test_initial_matches_oracle(small_case):
+    """initial(): kinematics of every mode plus one getnonlinear (PF:1932-1946)."""
+    cfg, p, (v, oy, au0, w0), ref = small_case
+    with make_channel(cfg) as ch:
+        ch.initial(v, oy, au0, w0)
+        got = ch.get_state()
+    errs = state_errors(got, ref)
+    report("initial", errs)
+    for k, e in errs.items():
+        assert e < ONE_OP_TOL, (k, e)
+
+
+def test_solve_matches_oracle(small_case):
+    """One per-mode solve + mean mode (PF:242-474) from the same pre-solve state."""
+    cfg, p, _, ref = small_case
+    st = O.copy_state(ref)
+    O.Stepper(p).solve(st)
+    with make_channel(cfg) as ch:
+        ch.set_state(ref)
+        ch.solve()
+        got = ch.get_state()
+        fl = ch.fields()
+    mz = p.Mz
+    errs = {k: rel_err(got[k], st[k]) for k in ("vnp1", "omegaynp1", "au0", "w0")}
+    for k in ("unp1", "wnp1", "omegaxnp1", "omegaznp1"):
+        g = fl[k].copy()
+        r = st[k].copy()
+        g[:, mz // 2, :] = 0     # the Nyquist column is never solved (PF:247)
+        r[:, mz // 2, :] = 0
+        errs[k] = rel_err(g, r)
+    report("solve", errs)
+    for k, e in errs.items():
+        assert e < ONE_OP_TOL, (k, e)
+
+
+def test_nonlinear_matches_oracle(small_case):
+    """getnonlinear (PF:1380-1612) on the fields left by one solve."""
+    cfg, p, _, ref = small_case
+    st = O.copy_state(ref)
+    O.Stepper(p).solve(st)
+    O.getnonlinear(st, p)
+    with make_channel(cfg) as ch:
+        ch.set_state(ref)
+        ch.solve()
+        ch.getnonlinear()
+        got = ch.get_state()
+    errs = state_errors(got, st)
+    report("nonlinear", errs)
+    for k, e in errs.items():
+        assert e < ONE_OP_TOL, (k, e)
+
+
+@pytest.mark.parametrize("nsteps", [10, 100])
+def test_cfg1_trajectory(require_gpu, nsteps):
+    """Config 1 (32x33x32, Re_c = 3000, dt = 1e-3): n steps against the oracle."""
+    cfg = ic.CONFIGS["cfg1_32x33x32"]
+    p = params_of(cfg)
+    v, oy, au0, w0 = ic_of(cfg)
+    st = O.initial_presolve_state(v, oy, au0, w0, p)
+    O.Stepper(p).step(st, nsteps)
+    with make_channel(cfg) as ch:
+        ch.initial(v, oy, au0, w0)
+        ch.step(nsteps)
+        got = ch.get_state()
+    errs = state_errors(got, st)
+    report(f"cfg1 {nsteps} steps", errs)
+    for k, e in errs.items():
+        assert e < TRAJ_TOL, (k, e)
+
+
+def test_odd_degree_and_rectangular_grid(require_gpu):
+    """N odd, nnx != nny, aphi/beta != 1: 3 steps against the oracle."""
+    cfg = dict(nnx=24, nny=16, nnz=21, aphi=2.0, beta=2.0 / 3.0, re=180.0, re_p=1.0, dt=5.0e-4,
+               u_c=18.0, rms=1.8, kmax=6.0, slope=-4.0 / 3.0, seed=11)
+    p = params_of(cfg)
+    v, oy, au0, w0 = ic_of(cfg)
+    st = O.initial_presolve_state(v, oy, au0, w0, p)
+    O.Stepper(p).step(st, 3)
+    with make_channel(cfg) as ch:
+        ch.initial(v, oy, au0, w0)
+        ch.step(3)
+        got = ch.get_state()
+    errs = state_errors(got, st)
+    report("odd N", errs)
+    for k, e in errs.items():
+        assert e < TRAJ_TOL, (k, e)
+
+
+def test_laminar_fixed_point_gpu(require_gpu):
+    """U = (re re_p/2)(1-y^2) with steady AB2 history is unchanged after 20 steps."""
+    cfg = dict(SMALL)
+    p = params_of(cfg)
+    z = np.zeros((p.Ny, p.Mz, p.Mx), complex)
+    au0 = np.zeros(p.Ny, complex)
+    au0[0], au0[2] = 0.5, -0.5
+    ref = O.initial_presolve_state(z, z, au0, np.zeros(p.Ny, complex), p)
+    ref["clamb1nm1"] = ref["clamb1n"].copy()
+    with make_channel(cfg) as ch:
+        ch.set_state(ref)
+        ch.step(20)
+        got = ch.get_state()
+    assert rel_err(got["au0"], au0) < 1e-13
+    assert np.max(np.abs(got["vnp1"])) == 0.0
