@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "chf_internal.h"
@@ -85,6 +86,29 @@ hipError_t dalloc(Ctx* c, T** p, size_t n) {
   return e;
 }
 
+// Packs NACC matrices A_m(r, k) (M x K) into MFMA A-operand fragment order
+// (frag[rb][kb][lane] = A_m[16 rb + lane%16][4 kb + lane/16]) and uploads them.
+hipError_t pack_op(Ctx* c, ChebOp& op, int M, int K, int nacc,
+                   const std::function<double(int, int, int)>& A) {
+  op.sh = mfma_shape(M, K);
+  const size_t n = (size_t)op.sh.MB * op.sh.KB * 64;
+  std::vector<double> f(n);
+  for (int m = 0; m < nacc; ++m) {
+    std::fill(f.begin(), f.end(), 0.0);
+    for (int rb = 0; rb < op.sh.MB; ++rb)
+      for (int kb = 0; kb < op.sh.KB; ++kb)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int r = rb * 16 + (lane & 15), k = kb * 4 + (lane >> 4);
+          if (r < M && k < K) f[((size_t)rb * op.sh.KB + kb) * 64 + lane] = A(m, r, k);
+        }
+    hipError_t e = dalloc(c, &op.frag[m], n);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(op.frag[m], f.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 int validate(const chf_params* p, std::string& msg) {
   if (!p) { msg = "null parameters"; return -1; }
   if (p->nnx < 4 || p->nnx % 4) { msg = "nnx must be a positive multiple of 4 (PF:721-765 padding)"; return -1; }
@@ -117,7 +141,9 @@ void free_all(Ctx* c) {
   f(c->hvc); f(c->hgc);
   f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
-  f(c->invV); f(c->invW); f(c->twx); f(c->twz);
+  f(c->invV); f(c->invW); f(c->twx); f(c->twz); f(c->costab);
+  for (ChebOp* op : {&c->op_rv, &c->op_ro})
+    for (double*& q : op->frag) { f(q); q = nullptr; }
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -224,6 +250,27 @@ int build(Ctx* c) {
   CHF_TRY(hipMemcpy(c->tsyn, ts.data(), ts.size() * sizeof(double), hipMemcpyHostToDevice));
   CHF_TRY(hipMemcpy(c->tana, ta.data(), ta.size() * sizeof(double), hipMemcpyHostToDevice));
   CHF_TRY(hipMemcpy(c->tpts, tp.data(), tp.size() * sizeof(double), hipMemcpyHostToDevice));
+
+  // contraction operators packed for the matrix cores (even/odd degree split)
+  const int Ny = d.Ny, ne = d.ne, no = d.no;
+  {
+    std::vector<double> ct(4 * (size_t)N);
+    for (int n = 0; n < 4 * N; ++n) ct[n] = cospi_frac(n, 2L * N);
+    CHF_TRY(dalloc(c, &c->costab, ct.size()));
+    CHF_TRY(hipMemcpy(c->costab, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  CHF_TRY(pack_op(c, c->op_rv, ne - 2, ne, 6, [&](int m, int r, int j) {
+    const int par = m / 3, which = m % 3, l = 2 * j + par;
+    if (r >= (par ? no - 2 : ne - 2) || l > N) return 0.0;
+    const std::vector<double>& T = which == 0 ? t4 : (which == 1 ? t2 : t0);
+    return T[(size_t)(2 + r) * Ny + l];
+  }));
+  CHF_TRY(pack_op(c, c->op_ro, ne - 1, ne, 4, [&](int m, int r, int j) {
+    const int par = m / 2, which = m % 2, l = 2 * j + par;
+    if (r >= (par ? no - 1 : ne - 1) || l > N) return 0.0;
+    const std::vector<double>& T = which == 0 ? t2 : t0;
+    return T[(size_t)(1 + r) * Ny + l];
+  }));
 
   // FFT twiddles e^{-2 pi i m/L}
   auto twiddles = [](int L) {

@@ -17,169 +17,130 @@
 //   hassemble    : h_v = k^2 L_y + i alpha D L_x + i beta D L_z,
 //                  h_g = i alpha L_z - i beta L_x (PF:1502-1540, dz PF:570-599)
 #include "chf_internal.h"
+#include "mfma.h"
 
 namespace chf {
 
-// Generic column kernel: each thread owns one real column and TM rows of
-// NACC accumulators; the A rows of the block are staged in LDS and read as
-// broadcasts.
-template <class Prob, int TM>
-__global__ __launch_bounds__(256) void k_cols(Prob pb) {
-  constexpr int NA = Prob::NACC;
-  extern __shared__ double sA[];
-  Prob p = pb;
-  p.select(blockIdx.z);
-  const int K = p.K;
-  const int r0 = blockIdx.y * TM;
-  for (int i = threadIdx.x; i < NA * TM * K; i += blockDim.x) {
-    const int m = i / (TM * K);
-    const int rem = i - m * TM * K;
-    const int r = rem / K;
-    const int j = rem - r * K;
-    sA[i] = p.a(m, r0 + r, j);
-  }
-  __syncthreads();
-  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= p.ncold) return;
-  double acc[NA][TM];
-#pragma unroll
-  for (int m = 0; m < NA; ++m)
-#pragma unroll
-    for (int r = 0; r < TM; ++r) acc[m][r] = 0.0;
-  for (int j = 0; j < K; ++j) {
-    double b[NA];
-    p.loadB(j, col, b);
-#pragma unroll
-    for (int m = 0; m < NA; ++m)
-#pragma unroll
-      for (int r = 0; r < TM; ++r) acc[m][r] = fma(sA[(m * TM + r) * K + j], b[m], acc[m][r]);
-  }
-#pragma unroll
-  for (int r = 0; r < TM; ++r) {
-    double v[NA];
-#pragma unroll
-    for (int m = 0; m < NA; ++m) v[m] = acc[m][r];
-    p.store(r0 + r, col, v);
-  }
-}
-
-template <class Prob, int TM>
-static hipError_t run_cols(Ctx& c, const Prob& pb, int rows, int nz) {
-  dim3 grid((unsigned)((pb.ncold + 255) / 256), (unsigned)((rows + TM - 1) / TM), nz);
-  size_t shm = sizeof(double) * Prob::NACC * TM * pb.K;
-  hipLaunchKernelGGL((k_cols<Prob, TM>), grid, dim3(256), shm, c.stream, pb);
-  return hipGetLastError();
-}
-
 // ---------------------------------------------------------------- synthesis
-// out[p] = E + O, out[P-1-p] = E - O for p < Ph, E/O = even/odd degree sums.
-struct SynthProb {
-  static constexpr int NACC = 2;
-  const double* A;  // [Ph][Ny]
+// out[p] = E + O, out[P-1-p] = E - O for p < Ph, E/O = even/odd degree sums,
+// with A_m(p, j) = T_{2j+m}(y_p) = cos(pi f p (2j+m) / 2N) (f = 1 on the
+// 2N+1-point grid, f = 2 on the N+1-point grid).
+struct SynthP {
+  static constexpr int NACC = 2, NB = 2;
+  static constexpr int bsel(int m) { return m; }
+  const double* costab;
+  int n4;        // 4N
+  int f;         // grid factor
+  MfmaShape sh;  // M = folded points, K = ne
   const double* inf[6];
   double* outf[6];
   const double* in;
   double* out;
-  int Ny, Ph, P, K;
+  int Ny, P;
   long ncold;
   __device__ void select(int z) {
     in = inf[z];
     out = outf[z];
   }
-  __device__ double a(int m, int r, int j) const {
-    const int l = 2 * j + m;
-    return (r < Ph && l < Ny) ? A[r * Ny + l] : 0.0;
+  __device__ int aindex(int m, int row, int k0, int n4_) const {
+    return (int)(((long)f * row * (2 * k0 + m)) % n4_);
   }
-  __device__ void loadB(int j, long col, double* b) const {
+  __device__ int astep(int, int row, int n4_) const { return (int)((8L * f * row) % n4_); }
+  __device__ void loadB(int j, long col, double* v) const {
     const int l = 2 * j;
-    b[0] = in[l * ncold + col];
-    b[1] = (l + 1 < Ny) ? in[(l + 1) * ncold + col] : 0.0;
+    v[0] = in[l * ncold + col];
+    v[1] = (l + 1 < Ny) ? in[(l + 1) * ncold + col] : 0.0;
   }
   __device__ void store(int r, long col, const double* acc) const {
-    if (r >= Ph) return;
     out[r * ncold + col] = acc[0] + acc[1];
     const int q = P - 1 - r;
     if (q != r) out[q * ncold + col] = acc[0] - acc[1];
   }
 };
 
-static hipError_t synth(Ctx& c, const double* A, int Ph, int P, const double2* const* in,
-                        double2* const* out, int nf) {
-  SynthProb pb{};
-  pb.A = A;
-  for (int f = 0; f < nf; ++f) {
-    pb.inf[f] = reinterpret_cast<const double*>(in[f]);
-    pb.outf[f] = reinterpret_cast<double*>(out[f]);
+static hipError_t synth(Ctx& c, int f, int Ph, int P, const double2* const* in, double2* const* out,
+                        int nf) {
+  SynthP pb{};
+  pb.costab = c.costab;
+  pb.n4 = 4 * c.d.N;
+  pb.f = f;
+  pb.sh = mfma_shape(Ph, c.d.ne);
+  for (int i = 0; i < nf; ++i) {
+    pb.inf[i] = reinterpret_cast<const double*>(in[i]);
+    pb.outf[i] = reinterpret_cast<double*>(out[i]);
   }
   pb.Ny = c.d.Ny;
-  pb.Ph = Ph;
   pb.P = P;
-  pb.K = c.d.ne;
   pb.ncold = 2 * c.d.ncol;
-  return run_cols<SynthProb, 16>(c, pb, Ph, nf);
+  return run_mfma_cos(c.stream, pb, nf);
 }
 
 hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return synth(c, c.tsyn, c.d.Ph, c.d.Py, in, out, nf);
+  return synth(c, 1, c.d.Ph, c.d.Py, in, out, nf);
 }
 
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return synth(c, c.tpts, c.d.Pn, c.d.Ny, in, out, nf);
+  return synth(c, 2, c.d.Pn, c.d.Ny, in, out, nf);
 }
 
 // ----------------------------------------------------------------- analysis
-// coefficient 2j (2j+1) = sum over folded points of W[k][p] (x_p +- x_{P-1-p}).
-struct AnalProb {
-  static constexpr int NACC = 2;
-  const double* A;  // [Ny][Ph]
+// coefficient k = 2j+m: (1/(2N c_k)) sum_p w_p cos(pi k p/2N) (x_p +- x_{2N-p})
+// over the folded points p = 0..N (PF:1686-1700: /2N, c_0 = 2, w_0 = 1, else 2).
+struct AnalP {
+  static constexpr int NACC = 2, NB = 2;
+  static constexpr int bsel(int m) { return m; }
+  const double* costab;
+  int n4;
+  MfmaShape sh;  // M = ne, K = Ph
   const double* inf[6];
   double* outf[6];
   const double* in;
   double* out;
-  int Ny, Ph, P, K, N;
+  int P, N;
   long ncold;
+  double inv2n;
   __device__ void select(int z) {
     in = inf[z];
     out = outf[z];
   }
-  __device__ double a(int m, int r, int p) const {
-    const int k = 2 * r + m;
-    return (k <= N) ? A[k * Ph + p] : 0.0;
+  __device__ int aindex(int m, int row, int k0, int n4_) const {
+    return (int)(((long)(2 * row + m) * k0) % n4_);
   }
-  __device__ void loadB(int p, long col, double* b) const {
+  __device__ int astep(int m, int row, int n4_) const { return (int)((4L * (2 * row + m)) % n4_); }
+  __device__ void loadB(int p, long col, double* v) const {
     const double x = in[p * ncold + col];
     const int q = P - 1 - p;
+    const double w = (p == 0) ? 1.0 : 2.0;
     if (q == p) {
-      b[0] = x;
-      b[1] = 0.0;
+      v[0] = w * x;
+      v[1] = 0.0;
     } else {
       const double y = in[q * ncold + col];
-      b[0] = x + y;
-      b[1] = x - y;
+      v[0] = w * (x + y);
+      v[1] = w * (x - y);
     }
   }
   __device__ void store(int r, long col, const double* acc) const {
     const int k = 2 * r;
-    if (k > N) return;
-    out[k * ncold + col] = acc[0];
-    if (k + 1 <= N) out[(k + 1) * ncold + col] = acc[1];
+    out[k * ncold + col] = acc[0] * (k == 0 ? 0.5 * inv2n : inv2n);
+    if (k + 1 <= N) out[(k + 1) * ncold + col] = acc[1] * inv2n;
   }
 };
 
 hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  AnalProb pb{};
-  pb.A = c.tana;
-  for (int f = 0; f < nf; ++f) {
-    pb.inf[f] = reinterpret_cast<const double*>(in[f]);
-    pb.outf[f] = reinterpret_cast<double*>(out[f]);
+  AnalP pb{};
+  pb.costab = c.costab;
+  pb.n4 = 4 * c.d.N;
+  pb.sh = mfma_shape(c.d.ne, c.d.Ph);
+  for (int i = 0; i < nf; ++i) {
+    pb.inf[i] = reinterpret_cast<const double*>(in[i]);
+    pb.outf[i] = reinterpret_cast<double*>(out[i]);
   }
-  pb.Ny = c.d.Ny;
-  pb.Ph = c.d.Ph;
   pb.P = c.d.Py;
-  pb.K = c.d.Ph;
   pb.N = c.d.N;
   pb.ncold = 2 * c.d.ncol;
-  return run_cols<AnalProb, 16>(c, pb, c.d.ne, nf);
+  pb.inv2n = 1.0 / (2.0 * c.d.N);
+  return run_mfma_cos(c.stream, pb, nf);
 }
 
 // ----------------------------------------------------------------------- rhs
@@ -192,34 +153,24 @@ __device__ __forceinline__ double col_k2(long col, int Mx, const double* al, con
 }
 
 // v equation rows at points p = 2 + r: c T4 + (1 - k^2 dt/re) T2 - k^2 (1 - k^2 c) T0
-struct RhsVProb {
-  static constexpr int NACC = 6;  // (E: T4, T2, T0), (O: T4, T2, T0)
-  const double *t0, *t2, *t4;
+struct RhsVP {
+  static constexpr int NACC = 6, NB = 2;  // (E: T4, T2, T0), (O: T4, T2, T0)
+  static constexpr int bsel(int m) { return m / 3; }
+  const double* frag[6];
+  MfmaShape sh;  // M = ne - 2, K = ne
   const double* vin;
   const double *hn, *hp;
   double *outE, *outO;
   const double *al, *be;
-  int Ny, N, K, ne, no, Mx;
+  int N, ne, no, Mx;
   long ncold;
   double dt, re;
   __device__ void select(int) {}
-  __device__ double a(int m, int r, int j) const {
-    const int par = m / 3, which = m - 3 * par;
-    const int l = 2 * j + par;
-    const int nrow = par ? no - 2 : ne - 2;
-    if (r >= nrow || l > N) return 0.0;
-    const int p = 2 + r;
-    const double* T = which == 0 ? t4 : (which == 1 ? t2 : t0);
-    return T[p * Ny + l];
-  }
-  __device__ void loadB(int j, long col, double* b) const {
-    const double e = vin[(2 * j) * ncold + col];
-    const double o = (2 * j + 1 <= N) ? vin[(2 * j + 1) * ncold + col] : 0.0;
-    b[0] = b[1] = b[2] = e;
-    b[3] = b[4] = b[5] = o;
+  __device__ void loadB(int j, long col, double* v) const {
+    v[0] = vin[(2 * j) * ncold + col];
+    v[1] = (2 * j + 1 <= N) ? vin[(2 * j + 1) * ncold + col] : 0.0;
   }
   __device__ void store(int r, long col, const double* acc) const {
-    if (r >= ne - 2) return;  // ne >= no: no row of either half lies beyond
     const double k2 = col_k2(col, Mx, al, be);
     const double c = dt * 0.5 / re;
     const double c2 = 1.0 - k2 * dt / re;
@@ -227,53 +178,39 @@ struct RhsVProb {
     const int p = 2 + r, q = N - p;
     const double fp = 3.0 * dt / 2.0 * hn[p * ncold + col] - dt / 2.0 * hp[p * ncold + col];
     const double fq = 3.0 * dt / 2.0 * hn[q * ncold + col] - dt / 2.0 * hp[q * ncold + col];
-    if (r < ne - 2) {
-      const double sym = (p == q) ? fp : 0.5 * (fp + fq);
-      outE[r * ncold + col] = c * acc[0] + c2 * acc[1] - c0 * acc[2] + sym;
-    }
+    const double sym = (p == q) ? fp : 0.5 * (fp + fq);
+    outE[r * ncold + col] = c * acc[0] + c2 * acc[1] - c0 * acc[2] + sym;
     if (r < no - 2) outO[r * ncold + col] = c * acc[3] + c2 * acc[4] - c0 * acc[5] + 0.5 * (fp - fq);
   }
 };
 
 // omega_y rows at points p = 1 + r: c T2 + (1 - k^2 c) T0
-struct RhsOProb {
-  static constexpr int NACC = 4;  // (E: T2, T0), (O: T2, T0)
-  const double *t0, *t2;
+struct RhsOP {
+  static constexpr int NACC = 4, NB = 2;  // (E: T2, T0), (O: T2, T0)
+  static constexpr int bsel(int m) { return m / 2; }
+  const double* frag[4];
+  MfmaShape sh;  // M = ne - 1, K = ne
   const double* oin;
   const double *hn, *hp;
   double *outE, *outO;
   const double *al, *be;
-  int Ny, N, K, ne, no, Mx;
+  int N, ne, no, Mx;
   long ncold;
   double dt, re;
   __device__ void select(int) {}
-  __device__ double a(int m, int r, int j) const {
-    const int par = m / 2, which = m - 2 * par;
-    const int l = 2 * j + par;
-    const int nrow = par ? no - 1 : ne - 1;
-    if (r >= nrow || l > N) return 0.0;
-    const int p = 1 + r;
-    const double* T = which == 0 ? t2 : t0;
-    return T[p * Ny + l];
-  }
-  __device__ void loadB(int j, long col, double* b) const {
-    const double e = oin[(2 * j) * ncold + col];
-    const double o = (2 * j + 1 <= N) ? oin[(2 * j + 1) * ncold + col] : 0.0;
-    b[0] = b[1] = e;
-    b[2] = b[3] = o;
+  __device__ void loadB(int j, long col, double* v) const {
+    v[0] = oin[(2 * j) * ncold + col];
+    v[1] = (2 * j + 1 <= N) ? oin[(2 * j + 1) * ncold + col] : 0.0;
   }
   __device__ void store(int r, long col, const double* acc) const {
-    if (r >= ne - 1) return;
     const double k2 = col_k2(col, Mx, al, be);
     const double c = dt * 0.5 / re;
     const double c0 = 1.0 - k2 * c;
     const int p = 1 + r, q = N - p;
     const double fp = 3.0 * dt / 2.0 * hn[p * ncold + col] - dt / 2.0 * hp[p * ncold + col];
     const double fq = 3.0 * dt / 2.0 * hn[q * ncold + col] - dt / 2.0 * hp[q * ncold + col];
-    if (r < ne - 1) {
-      const double sym = (p == q) ? fp : 0.5 * (fp + fq);
-      outE[r * ncold + col] = c * acc[0] + c0 * acc[1] + sym;
-    }
+    const double sym = (p == q) ? fp : 0.5 * (fp + fq);
+    outE[r * ncold + col] = c * acc[0] + c0 * acc[1] + sym;
     if (r < no - 1) outO[r * ncold + col] = c * acc[2] + c0 * acc[3] + 0.5 * (fp - fq);
   }
 };
@@ -283,10 +220,9 @@ hipError_t launch_rhs(Ctx& c, int which) {
   const int nxt = c.hcur;        // h^n
   const int prv = 1 - c.hcur;    // h^{n-1}
   if (which == 0) {
-    RhsVProb pv{};
-    pv.t0 = c.t0;
-    pv.t2 = c.t2;
-    pv.t4 = c.t4;
+    RhsVP pv{};
+    for (int m = 0; m < 6; ++m) pv.frag[m] = c.op_rv.frag[m];
+    pv.sh = c.op_rv.sh;
     pv.vin = reinterpret_cast<const double*>(c.v);
     pv.hn = reinterpret_cast<const double*>(c.hv[nxt]);
     pv.hp = reinterpret_cast<const double*>(c.hv[prv]);
@@ -294,20 +230,18 @@ hipError_t launch_rhs(Ctx& c, int which) {
     pv.outO = reinterpret_cast<double*>(c.rvO);
     pv.al = c.alpha;
     pv.be = c.beta;
-    pv.Ny = d.Ny;
     pv.N = d.N;
-    pv.K = d.ne;
     pv.ne = d.ne;
     pv.no = d.no;
     pv.Mx = d.Mx;
     pv.ncold = 2 * d.ncol;
     pv.dt = c.prm.dt;
     pv.re = c.prm.re;
-    return run_cols<RhsVProb, 8>(c, pv, d.ne - 2, 1);
+    return run_mfma(c.stream, pv, 1);
   }
-  RhsOProb po{};
-  po.t0 = c.t0;
-  po.t2 = c.t2;
+  RhsOP po{};
+  for (int m = 0; m < 4; ++m) po.frag[m] = c.op_ro.frag[m];
+  po.sh = c.op_ro.sh;
   po.oin = reinterpret_cast<const double*>(c.oy);
   po.hn = reinterpret_cast<const double*>(c.hg[nxt]);
   po.hp = reinterpret_cast<const double*>(c.hg[prv]);
@@ -315,16 +249,14 @@ hipError_t launch_rhs(Ctx& c, int which) {
   po.outO = reinterpret_cast<double*>(c.roO);
   po.al = c.alpha;
   po.be = c.beta;
-  po.Ny = d.Ny;
   po.N = d.N;
-  po.K = d.ne;
   po.ne = d.ne;
   po.no = d.no;
   po.Mx = d.Mx;
   po.ncold = 2 * d.ncol;
   po.dt = c.prm.dt;
   po.re = c.prm.re;
-  return run_cols<RhsOProb, 8>(c, po, d.ne - 1, 1);
+  return run_mfma(c.stream, po, 1);
 }
 
 // ---------------------------------------------------------------- h assembly

@@ -38,6 +38,28 @@ struct Mats {
   const double* t4;     // [Ny][Ny]   SCHEB T4
 };
 
+// Shape of a contraction on FP64 matrix cores (16x16x4 blocks).
+struct MfmaShape {
+  int M, K;   // logical rows / reduction length
+  int MB;     // row blocks of 16
+  int KB;     // k blocks of 4
+};
+
+inline MfmaShape mfma_shape(int M, int K) {
+  MfmaShape s{};
+  s.M = M;
+  s.K = K;
+  s.MB = (M + 15) / 16;
+  s.KB = (K + 3) / 4;
+  return s;
+}
+
+// A wall-normal operator packed as MFMA A-operand fragments, one per accumulator.
+struct ChebOp {
+  double* frag[6] = {};
+  MfmaShape sh{};
+};
+
 struct Ctx {
   chf_params prm{};
   Dims d{};
@@ -66,6 +88,8 @@ struct Ctx {
   double* host_mats = nullptr;
   double *tsyn = nullptr, *tana = nullptr, *tpts = nullptr, *t0 = nullptr, *t2 = nullptr,
          *t4 = nullptr;
+  ChebOp op_rv, op_ro;     // packed explicit-operator fragments (rhs)
+  double* costab = nullptr;  // cos(pi n / 2N), n < 4N
   double* invV = nullptr;  // [ngroups][ne^2 + no^2] transposed inverses (v)
   double* invW = nullptr;  // [ngroups][ne^2 + no^2] (omega_y; group 0 = mean mode)
   double2* twx = nullptr;  // exp(-2 pi i m/Lx), m < Lx

@@ -1,0 +1,267 @@
+// Generic wall-normal contraction on FP64 matrix cores.
+//
+//   out = epilogue( A_m . B_{s(m)} , m = 0..NACC-1 )
+//
+// A_m: small real matrices [M x K] (row = output row), pre-packed on the host
+//      in v_mfma_f64_16x16x4f64 operand order: frag[m][rb][kb][lane] =
+//      A_m[16 rb + lane%16][4 kb + lane/16];
+// B_s: NB real column streams [K x ncold] produced by Prob::loadB from the
+//      field arrays (parity gathers, mirrored-point folds);
+// one workgroup owns all M rows of a BN-column tile, so every B element is
+// read from HBM once; wave w owns row blocks w, w+4, w+8, ...
+#pragma once
+#include "chf_internal.h"
+
+namespace chf {
+
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+template <class P, int RBW, int CB>
+__global__ __launch_bounds__(256) void k_mfma(P pb) {
+  constexpr int NA = P::NACC, NB = P::NB, KC = 16, BN = 16 * CB;
+  constexpr int SB = (BN % 32 == 0) ? BN + 16 : BN;  // row stride: 2 rows per half-wave hit disjoint banks
+  __shared__ double sB[NB][KC][SB];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  P p = pb;
+  p.select(blockIdx.z);
+  const long col0 = (long)blockIdx.x * BN;
+  dvec4 acc[NA][RBW][CB];
+#pragma unroll
+  for (int m = 0; m < NA; ++m)
+#pragma unroll
+    for (int r = 0; r < RBW; ++r)
+#pragma unroll
+      for (int c = 0; c < CB; ++c) acc[m][r][c] = dvec4{0.0, 0.0, 0.0, 0.0};
+  const int MB = p.sh.MB, KB = p.sh.KB;
+  const int nkc = (KB * 4 + KC - 1) / KC;
+  for (int kc = 0; kc < nkc; ++kc) {
+    for (int i = threadIdx.x; i < KC * BN; i += 256) {
+      const int kk = i / BN, c = i - kk * BN;
+      const int k = kc * KC + kk;
+      const long col = col0 + c;
+      double v[NB];
+      if (k < p.sh.K && col < p.ncold) {
+        p.loadB(k, col, v);
+      } else {
+#pragma unroll
+        for (int s = 0; s < NB; ++s) v[s] = 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < NB; ++s) sB[s][kk][c] = v[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KC / 4; ++kb) {
+      const int kg = kc * (KC / 4) + kb;
+      if (kg < KB) {
+        double bv[NB][CB];
+#pragma unroll
+        for (int s = 0; s < NB; ++s)
+#pragma unroll
+          for (int c = 0; c < CB; ++c) bv[s][c] = sB[s][kb * 4 + (lane >> 4)][c * 16 + (lane & 15)];
+#pragma unroll
+        for (int r = 0; r < RBW; ++r) {
+          const int rb = wave + 4 * r;
+          if (rb < MB) {
+#pragma unroll
+            for (int m = 0; m < NA; ++m) {
+              const double a = p.frag[m][((long)rb * KB + kg) * 64 + lane];
+#pragma unroll
+              for (int c = 0; c < CB; ++c)
+                acc[m][r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[P::bsel(m)][c], acc[m][r][c], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < RBW; ++r) {
+    const int rb = wave + 4 * r;
+    if (rb >= MB) continue;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const long col = col0 + c * 16 + (lane & 15);
+      if (col >= p.ncold) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // v_mfma_f64_16x16x4f64 result: lane l, element e holds D[l/16 + 4e][l%16]
+        const int row = rb * 16 + (lane >> 4) + 4 * e;
+        if (row >= p.sh.M) continue;
+        double v[NA];
+#pragma unroll
+        for (int m = 0; m < NA; ++m) v[m] = acc[m][r][c][e];
+        p.store(row, col, v);
+      }
+    }
+  }
+}
+
+// Variant for the transform contractions, whose matrix entries are all of the
+// form cos(pi n / 2N): A_m(r, k) = costab[(r0(m, r) + k * dk(m, r)) mod 4N],
+// read from a 4N-entry table in LDS through indices that each lane advances
+// incrementally per k block, so no matrix bytes move from HBM or L2.  B tiles
+// are software-pipelined: the next tile is loaded into registers while the
+// current one feeds the matrix cores.
+template <class P, int RBW, int CB>
+__global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
+  constexpr int NA = P::NACC, NB = P::NB, KC = 16, BN = 16 * CB;
+  constexpr int SB = (BN % 32 == 0) ? BN + 16 : BN;
+  constexpr int PER = (KC * BN + 255) / 256;  // tile elements per thread
+  __shared__ double sB[NB][KC][SB];
+  extern __shared__ double stab[];           // 4N cosines
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  P p = pb;
+  p.select(blockIdx.z);
+  const int n4 = p.n4;
+  for (int i = threadIdx.x; i < n4; i += 256) stab[i] = p.costab[i];
+  const long col0 = (long)blockIdx.x * BN;
+  const int MB = p.sh.MB, KB = p.sh.KB;
+  // per (m, r): current table index and its per-k-block increment
+  int idx[NA][RBW], stp[NA][RBW];
+#pragma unroll
+  for (int m = 0; m < NA; ++m)
+#pragma unroll
+    for (int r = 0; r < RBW; ++r) {
+      const int row = (wave + 4 * r) * 16 + (lane & 15);
+      const int k0 = lane >> 4;
+      idx[m][r] = p.aindex(m, row, k0, n4);
+      stp[m][r] = p.astep(m, row, n4);  // index advance for k -> k + 4
+    }
+  dvec4 acc[NA][RBW][CB];
+#pragma unroll
+  for (int m = 0; m < NA; ++m)
+#pragma unroll
+    for (int r = 0; r < RBW; ++r)
+#pragma unroll
+      for (int c = 0; c < CB; ++c) acc[m][r][c] = dvec4{0.0, 0.0, 0.0, 0.0};
+  const int nkc = (KB * 4 + KC - 1) / KC;
+  double nxt[PER][NB];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = threadIdx.x + q * 256;
+      const int kk = i / BN, c = i - kk * BN;
+      const int k = kc * KC + kk;
+      const long col = col0 + c;
+      if (i < KC * BN && k < p.sh.K && col < p.ncold) {
+        p.loadB(k, col, nxt[q]);
+      } else {
+#pragma unroll
+        for (int s = 0; s < NB; ++s) nxt[q][s] = 0.0;
+      }
+    }
+  };
+  fetch(0);
+  for (int kc = 0; kc < nkc; ++kc) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int i = threadIdx.x + q * 256;
+      if (i < KC * BN) {
+        const int kk = i / BN, c = i - kk * BN;
+#pragma unroll
+        for (int s = 0; s < NB; ++s) sB[s][kk][c] = nxt[q][s];
+      }
+    }
+    __syncthreads();
+    if (kc + 1 < nkc) fetch(kc + 1);
+#pragma unroll
+    for (int kb = 0; kb < KC / 4; ++kb) {
+      const int kg = kc * (KC / 4) + kb;
+      if (kg < KB) {
+        double bv[NB][CB];
+#pragma unroll
+        for (int s = 0; s < NB; ++s)
+#pragma unroll
+          for (int c = 0; c < CB; ++c) bv[s][c] = sB[s][kb * 4 + (lane >> 4)][c * 16 + (lane & 15)];
+#pragma unroll
+        for (int r = 0; r < RBW; ++r) {
+          const int rb = wave + 4 * r;
+          if (rb < MB) {
+#pragma unroll
+            for (int m = 0; m < NA; ++m) {
+              const double a = stab[idx[m][r]];
+              int t = idx[m][r] + stp[m][r];
+              idx[m][r] = t >= n4 ? t - n4 : t;
+#pragma unroll
+              for (int c = 0; c < CB; ++c)
+                acc[m][r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[P::bsel(m)][c], acc[m][r][c], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < RBW; ++r) {
+    const int rb = wave + 4 * r;
+    if (rb >= MB) continue;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const long col = col0 + c * 16 + (lane & 15);
+      if (col >= p.ncold) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // v_mfma_f64_16x16x4f64 result: lane l, element e holds D[l/16 + 4e][l%16]
+        const int row = rb * 16 + (lane >> 4) + 4 * e;
+        if (row >= p.sh.M) continue;
+        double v[NA];
+#pragma unroll
+        for (int m = 0; m < NA; ++m) v[m] = acc[m][r][c][e];
+        p.store(row, col, v);
+      }
+    }
+  }
+}
+
+template <class P, int RBW>
+hipError_t run_mfma_cos_rbw(hipStream_t st, const P& p, int nz) {
+  // 8 accumulator blocks per wave keeps 2-3 waves per SIMD to hide LDS/HBM latency
+  constexpr int budget = 8 / (P::NACC * RBW) > 0 ? 8 / (P::NACC * RBW) : 1;
+  constexpr int CB = budget >= 4 ? 4 : (budget >= 2 ? 2 : 1);
+  const long tiles = (p.ncold + 16 * CB - 1) / (16 * CB);
+  hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB>), dim3((unsigned)tiles, 1, nz), dim3(256),
+                     sizeof(double) * p.n4, st, p);
+  return hipGetLastError();
+}
+
+template <class P>
+hipError_t run_mfma_cos(hipStream_t st, const P& p, int nz) {
+  const int rbw = (p.sh.MB + 3) / 4;
+  switch (rbw) {
+    case 1: return run_mfma_cos_rbw<P, 1>(st, p, nz);
+    case 2: return run_mfma_cos_rbw<P, 2>(st, p, nz);
+    case 3: return run_mfma_cos_rbw<P, 3>(st, p, nz);
+    case 4: return run_mfma_cos_rbw<P, 4>(st, p, nz);
+    case 5: return run_mfma_cos_rbw<P, 5>(st, p, nz);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Picks rows-per-wave (from M) and columns-per-wave so that the accumulators
+// stay within NACC * RBW * CB <= 16 (64 doubles per lane).
+template <class P, int RBW>
+hipError_t run_mfma_rbw(hipStream_t st, const P& p, int nz) {
+  constexpr int budget = 16 / (P::NACC * RBW);
+  constexpr int CB = budget >= 4 ? 4 : (budget >= 2 ? 2 : 1);
+  const long tiles = (p.ncold + 16 * CB - 1) / (16 * CB);
+  hipLaunchKernelGGL((k_mfma<P, RBW, CB>), dim3((unsigned)tiles, 1, nz), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+template <class P>
+hipError_t run_mfma(hipStream_t st, const P& p, int nz) {
+  const int rbw = (p.sh.MB + 3) / 4;
+  switch (rbw) {
+    case 1: return run_mfma_rbw<P, 1>(st, p, nz);
+    case 2: return run_mfma_rbw<P, 2>(st, p, nz);
+    case 3: return run_mfma_rbw<P, 3>(st, p, nz);
+    case 4: return run_mfma_rbw<P, 4>(st, p, nz);
+    case 5: return run_mfma_rbw<P, 5>(st, p, nz);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace chf
