@@ -165,7 +165,7 @@ int build(Ctx* c) {
   d.Pn = d.N / 2 + 1;
   d.ncol = (long)d.Mz * d.Mx;
   d.ngroups = d.Mx * (d.Mz / 2);
-  d.inv_stride = (long)d.ne * d.ne + (long)d.no * d.no;
+  d.inv_stride = (long)d.ne * (d.ne + (d.ne & 1)) + (long)d.no * (d.no + (d.no & 1));
 
   CHF_TRY(hipGetDevice(&c->device));
   int max_lds = 0;

@@ -25,7 +25,7 @@ struct Dims {
   int Pn;     // N+1-point grid folded by parity: N/2+1
   long ncol;  // complex columns per plane: Mz*Mx
   int ngroups;  // (kx,|kz|) operator groups: Mx*(Mz/2)
-  long inv_stride;  // doubles per group per equation: ne^2 + no^2
+  long inv_stride;  // doubles per group per equation: ne*neP + no*noP (even leading dims)
 };
 
 // Device-side view of the wall-normal matrices (all point-major, row = point).

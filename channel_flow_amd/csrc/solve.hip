@@ -51,9 +51,11 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
   const int kx = g % a.Mx, kz = g / a.Mx;
   const double k2 = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
   const int n = par ? a.no : a.ne;
-  double* out = (eq ? a.invW : a.invV) + (long)g * a.stride + (par ? (long)a.ne * a.ne : 0);
+  const int nP = n + (n & 1);  // even leading dimension: 16-byte aligned row pairs
+  const int neP = a.ne + (a.ne & 1);
+  double* out = (eq ? a.invW : a.invV) + (long)g * a.stride + (par ? (long)a.ne * neP : 0);
   if (eq == 0 && g == 0) {  // k^2 = 0 has no v system (PF:184, 247)
-    for (int i = threadIdx.x; i < n * n; i += blockDim.x) out[i] = 0.0;
+    for (int i = threadIdx.x; i < n * nP; i += blockDim.x) out[i] = 0.0;
     return;
   }
   double* A = sm;  // n x n row-major
@@ -123,10 +125,10 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     }
     __syncthreads();
   }
-  // transposed store: out[j*n + i] = Ainv[i][j]
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const int j = idx / n, i = idx - j * n;
-    out[idx] = A[i * n + j];
+  // transposed store: out[j*nP + i] = Ainv[i][j], zero in the padding row i = n
+  for (int idx = threadIdx.x; idx < n * nP; idx += blockDim.x) {
+    const int j = idx / nP, i = idx - j * nP;
+    out[idx] = (i < n) ? A[i * n + j] : 0.0;
   }
 }
 
@@ -202,15 +204,18 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
   const bool two = (kzab > 0) || mean;
   const int kz0 = kzab, kz1 = kzab ? a.Mz - kzab : 0;
   const long col0 = (long)kz0 * a.Mx + kx, col1 = (long)kz1 * a.Mx + kx;
-  // LDS layout: rhs [2 eq][4 vec][Ny] ; V,O,U,W,DV,DU,DW [4 vec][Ny] each
-  double* rb = sm;                   // 8 * Ny
-  double* sv = rb + 8 * Ny;
+  // LDS, 24 Ny doubles with aliasing by live range:
+  //   region A (8 Ny): rhs [2 eq][4 vec][Ny], then U, W [4 vec][Ny] each
+  //   region B (8 Ny): V, O (new coefficients)
+  //   region C (4 Ny): DV, then DW ; region D (4 Ny): DU
+  double* rb = sm;
+  double* su = sm;
+  double* sw = sm + 4 * Ny;
+  double* sv = sm + 8 * Ny;
   double* so = sv + 4 * Ny;
-  double* su = so + 4 * Ny;
-  double* sw = su + 4 * Ny;
-  double* sd1 = sw + 4 * Ny;
-  double* sd2 = sd1 + 4 * Ny;
-  double* sd3 = sd2 + 4 * Ny;
+  double* sd1 = sm + 16 * Ny;
+  double* sd2 = sd1;
+  double* sd3 = sm + 20 * Ny;
   const double c = a.dt * 0.5 / a.re;
 
   if (a.mode == 0) {
@@ -269,29 +274,53 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
       }
     }
     __syncthreads();
-    // ---- dense half-system inverses: x = Einv be, Oinv bo
-    for (int t = threadIdx.x; t < 2 * Ny; t += blockDim.x) {
-      const int eq = t / Ny, row = t - eq * Ny;
+    // ---- dense half-system inverses: x = Einv be, Oinv bo.  Thread = one row
+    // pair (2i, 2i+1) of one of the four half systems; 16-byte matrix loads.
+    const int neP = ne + (ne & 1), noP = no + (no & 1);
+    const int pe = neP / 2, po = noP / 2;           // row pairs per half system
+    for (int t = threadIdx.x; t < 2 * (pe + po); t += blockDim.x) {
+      const int eq = t / (pe + po);
       if (mean && eq == 0) continue;
-      const bool odd = row >= ne;
-      const int n = odd ? no : ne;
-      const int i = odd ? row - ne : row;
-      const double* M = (eq ? a.invW : a.invV) + (long)g * a.stride + (odd ? (long)ne * ne : 0);
+      const int rr = t - eq * (pe + po);
+      const bool odd = rr >= pe;
+      const int i2 = 2 * (odd ? rr - pe : rr);
+      const int n = odd ? no : ne, nP = odd ? noP : neP;
+      const double* M = (eq ? a.invW : a.invV) + (long)g * a.stride + (odd ? (long)ne * neP : 0) + i2;
       const double* rhs = rb + eq * 4 * Ny + (odd ? ne : 0);
-      double x0 = 0, x1 = 0, x2 = 0, x3 = 0;
-      for (int j = 0; j < n; ++j) {
-        const double m = M[(long)j * n + i];
-        x0 = fma(m, rhs[0 * Ny + j], x0);
-        x1 = fma(m, rhs[1 * Ny + j], x1);
-        x2 = fma(m, rhs[2 * Ny + j], x2);
-        x3 = fma(m, rhs[3 * Ny + j], x3);
+      double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      int j = 0;
+      for (; j + 4 <= n; j += 4) {
+        double2 m4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m4[u] = *reinterpret_cast<const double2*>(M + (long)(j + u) * nP);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double r = rhs[q * Ny + j + u];
+            x[q] = fma(m4[u].x, r, x[q]);
+            x[4 + q] = fma(m4[u].y, r, x[4 + q]);
+          }
+        }
       }
-      const int l = 2 * i + (odd ? 1 : 0);
+      for (; j < n; ++j) {
+        const double2 m = *reinterpret_cast<const double2*>(M + (long)j * nP);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double r = rhs[q * Ny + j];
+          x[q] = fma(m.x, r, x[q]);
+          x[4 + q] = fma(m.y, r, x[4 + q]);
+        }
+      }
       double* dst = eq ? so : sv;
-      dst[0 * Ny + l] = x0;
-      dst[1 * Ny + l] = x1;
-      dst[2 * Ny + l] = x2;
-      dst[3 * Ny + l] = x3;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = i2 + h;
+        if (i >= n) break;
+        const int l = 2 * i + (odd ? 1 : 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[q * Ny + l] = x[4 * h + q];
+      }
     }
     __syncthreads();
   } else {
@@ -437,7 +466,7 @@ hipError_t launch_solve(Ctx& c, int mode) {
   a.dt = c.prm.dt;
   a.re = c.prm.re;
   a.mode = mode;
-  size_t shm = sizeof(double) * 36ull * d.Ny;
+  size_t shm = sizeof(double) * 24ull * d.Ny;
   if (shm > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
