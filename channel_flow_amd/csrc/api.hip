@@ -123,7 +123,11 @@ int validate(const chf_params* p, std::string& msg) {
     return -1;
   }
   if (p->precision != 0 && p->precision != 1) { msg = "precision must be 0 (FP64) or 1 (mixed)"; return -1; }
-  if (p->precision == 1) { msg = "mixed precision is not built yet"; return -1; }
+  if (p->precision == 1 && (!reg_fft_length(3 * p->nnx / 2) || !reg_fft_length(3 * p->nny / 2) ||
+                            (p->nnx / 2) % 16)) {
+    msg = "mixed precision needs 3*nnx/2 and 3*nny/2 in {12,24,48,96,144,192,384} and nnx % 32 == 0";
+    return -1;
+  }
   return 0;
 }
 
@@ -137,6 +141,9 @@ void free_all(Ctx* c) {
   f(c->au0); f(c->w0);
   for (auto& y : c->Y) f(y);
   for (auto& z : c->Z) f(z);
+  for (auto& y : c->Yf) f(y);
+  for (auto& z : c->Zf) f(z);
+  f(c->costabf); f(c->twxf); f(c->twzf);
   for (auto& l : c->Lc) f(l);
   f(c->hvc); f(c->hgc);
   f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
@@ -199,8 +206,13 @@ int build(Ctx* c) {
   CHF_TRY(dalloc(c, &c->w0, d.Ny));
   const size_t ysz = (size_t)d.Py * d.ncol;
   const size_t zsz = (size_t)d.Py * d.Lz * d.Mx;
-  for (auto& y : c->Y) CHF_TRY(dalloc(c, &y, ysz));
-  for (auto& z : c->Z) CHF_TRY(dalloc(c, &z, zsz));
+  if (p.precision == 0) {
+    for (auto& y : c->Y) CHF_TRY(dalloc(c, &y, ysz));
+    for (auto& z : c->Z) CHF_TRY(dalloc(c, &z, zsz));
+  } else {
+    for (auto& y : c->Yf) CHF_TRY(dalloc(c, &y, ysz));
+    for (auto& z : c->Zf) CHF_TRY(dalloc(c, &z, zsz));
+  }
   for (auto& l : c->Lc) CHF_TRY(dalloc(c, &l, fld));
   CHF_TRY(dalloc(c, &c->hvc, fld));
   CHF_TRY(dalloc(c, &c->hgc, fld));
@@ -258,6 +270,9 @@ int build(Ctx* c) {
     for (int n = 0; n < 4 * N; ++n) ct[n] = cospi_frac(n, 2L * N);
     CHF_TRY(dalloc(c, &c->costab, ct.size()));
     CHF_TRY(hipMemcpy(c->costab, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<float> cf(ct.begin(), ct.end());
+    CHF_TRY(dalloc(c, &c->costabf, cf.size()));
+    CHF_TRY(hipMemcpy(c->costabf, cf.data(), cf.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   CHF_TRY(pack_op(c, c->op_rv, ne - 2, ne, 6, [&](int m, int r, int j) {
     const int par = m / 3, which = m % 3, l = 2 * j + par;
@@ -285,6 +300,13 @@ int build(Ctx* c) {
     CHF_TRY(dalloc(c, &c->twz, d.Lz));
     CHF_TRY(hipMemcpy(c->twx, tx.data(), d.Lx * sizeof(double2), hipMemcpyHostToDevice));
     CHF_TRY(hipMemcpy(c->twz, tz.data(), d.Lz * sizeof(double2), hipMemcpyHostToDevice));
+    std::vector<float2> txf(d.Lx), tzf(d.Lz);
+    for (int m = 0; m < d.Lx; ++m) txf[m] = make_float2((float)tx[m].x, (float)tx[m].y);
+    for (int m = 0; m < d.Lz; ++m) tzf[m] = make_float2((float)tz[m].x, (float)tz[m].y);
+    CHF_TRY(dalloc(c, &c->twxf, d.Lx));
+    CHF_TRY(dalloc(c, &c->twzf, d.Lz));
+    CHF_TRY(hipMemcpy(c->twxf, txf.data(), d.Lx * sizeof(float2), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->twzf, tzf.data(), d.Lz * sizeof(float2), hipMemcpyHostToDevice));
   }
 
   // zero the state
@@ -327,12 +349,24 @@ hipError_t timed(Ctx* c, int id, F&& launch) {
 int nonlinear(Ctx* c) {
   // six fields in the order the streamwise pass pairs them: (u,v) (w,ox) (oy,oz)
   const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
-  CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return launch_synth_padded(*c, in6, c->Y, 6); }));
-  CHF_TRY(timed(c, K_ZFFT_INV, [&] { return launch_zfft_inv(*c, c->Y, c->Z, 6); }));
-  CHF_TRY(timed(c, K_XPASS, [&] { return launch_xpass(*c); }));
-  CHF_TRY(timed(c, K_ZFFT_FWD, [&] { return launch_zfft_fwd(*c, c->Z, c->Y, 3); }));
-  const double2* y3[3] = {c->Y[0], c->Y[1], c->Y[2]};
-  CHF_TRY(timed(c, K_ANALYSIS, [&] { return launch_analysis(*c, y3, c->Lc, 3); }));
+  if (c->prm.precision == 0) {
+    CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return launch_synth_padded(*c, in6, c->Y, 6); }));
+    CHF_TRY(timed(c, K_ZFFT_INV, [&] { return launch_zfft_inv(*c, c->Y, c->Z, 6); }));
+    CHF_TRY(timed(c, K_XPASS, [&] { return launch_xpass(*c); }));
+    CHF_TRY(timed(c, K_ZFFT_FWD, [&] { return launch_zfft_fwd(*c, c->Z, c->Y, 3); }));
+    const double2* y3[3] = {c->Y[0], c->Y[1], c->Y[2]};
+    CHF_TRY(timed(c, K_ANALYSIS, [&] { return launch_analysis(*c, y3, c->Lc, 3); }));
+  } else {
+    // mixed form: FP32 wall-normal values, FP32 FFTs and products, FP64 coefficients out
+    const float2* y6[6] = {c->Yf[0], c->Yf[1], c->Yf[2], c->Yf[3], c->Yf[4], c->Yf[5]};
+    const float2* z3[3] = {c->Zf[0], c->Zf[1], c->Zf[2]};
+    const float2* y3[3] = {c->Yf[0], c->Yf[1], c->Yf[2]};
+    CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return launch_synth_padded32(*c, in6, c->Yf, 6); }));
+    CHF_TRY(timed(c, K_ZFFT_INV, [&] { return launch_zfft32(*c, y6, c->Zf, 6, true); }));
+    CHF_TRY(timed(c, K_XPASS, [&] { return launch_xpass32(*c); }));
+    CHF_TRY(timed(c, K_ZFFT_FWD, [&] { return launch_zfft32(*c, z3, c->Yf, 3, false); }));
+    CHF_TRY(timed(c, K_ANALYSIS, [&] { return launch_analysis32(*c, y3, c->Lc, 3); }));
+  }
   CHF_TRY(timed(c, K_HASSEMBLE, [&] { return launch_hassemble(*c); }));
   const int nxt = 1 - c->hcur;
   const double2* hc[2] = {c->hvc, c->hgc};

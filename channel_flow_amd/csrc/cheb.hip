@@ -25,17 +25,19 @@ namespace chf {
 // out[p] = E + O, out[P-1-p] = E - O for p < Ph, E/O = even/odd degree sums,
 // with A_m(p, j) = T_{2j+m}(y_p) = cos(pi f p (2j+m) / 2N) (f = 1 on the
 // 2N+1-point grid, f = 2 on the N+1-point grid).
+template <class TI, class TO, class TCc>
 struct SynthP {
+  using TC = TCc;
   static constexpr int NACC = 2, NB = 2;
   static constexpr int bsel(int m) { return m; }
-  const double* costab;
+  const TC* costab;
   int n4;        // 4N
   int f;         // grid factor
   MfmaShape sh;  // M = folded points, K = ne
-  const double* inf[6];
-  double* outf[6];
-  const double* in;
-  double* out;
+  const TI* inf[6];
+  TO* outf[6];
+  const TI* in;
+  TO* out;
   int Ny, P;
   long ncold;
   __device__ void select(int z) {
@@ -46,28 +48,29 @@ struct SynthP {
     return (int)(((long)f * row * (2 * k0 + m)) % n4_);
   }
   __device__ int astep(int, int row, int n4_) const { return (int)((8L * f * row) % n4_); }
-  __device__ void loadB(int j, long col, double* v) const {
+  __device__ void loadB(int j, long col, TC* v) const {
     const int l = 2 * j;
-    v[0] = in[l * ncold + col];
-    v[1] = (l + 1 < Ny) ? in[(l + 1) * ncold + col] : 0.0;
+    v[0] = (TC)in[l * ncold + col];
+    v[1] = (l + 1 < Ny) ? (TC)in[(l + 1) * ncold + col] : (TC)0;
   }
-  __device__ void store(int r, long col, const double* acc) const {
-    out[r * ncold + col] = acc[0] + acc[1];
+  __device__ void store(int r, long col, const TC* acc) const {
+    out[r * ncold + col] = (TO)(acc[0] + acc[1]);
     const int q = P - 1 - r;
-    if (q != r) out[q * ncold + col] = acc[0] - acc[1];
+    if (q != r) out[q * ncold + col] = (TO)(acc[0] - acc[1]);
   }
 };
 
-static hipError_t synth(Ctx& c, int f, int Ph, int P, const double2* const* in, double2* const* out,
-                        int nf) {
-  SynthP pb{};
-  pb.costab = c.costab;
+template <class TI, class TO, class TC>
+static hipError_t synth_t(Ctx& c, const TC* costab, int f, int Ph, int P, const TI* const* in,
+                          TO* const* out, int nf) {
+  SynthP<TI, TO, TC> pb{};
+  pb.costab = costab;
   pb.n4 = 4 * c.d.N;
   pb.f = f;
   pb.sh = mfma_shape(Ph, c.d.ne);
   for (int i = 0; i < nf; ++i) {
-    pb.inf[i] = reinterpret_cast<const double*>(in[i]);
-    pb.outf[i] = reinterpret_cast<double*>(out[i]);
+    pb.inf[i] = in[i];
+    pb.outf[i] = out[i];
   }
   pb.Ny = c.d.Ny;
   pb.P = P;
@@ -75,27 +78,50 @@ static hipError_t synth(Ctx& c, int f, int Ph, int P, const double2* const* in, 
   return run_mfma_cos(c.stream, pb, nf);
 }
 
+static void as_real(const double2* const* in, const double** out, int nf) {
+  for (int i = 0; i < nf; ++i) out[i] = reinterpret_cast<const double*>(in[i]);
+}
+
 hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return synth(c, 1, c.d.Ph, c.d.Py, in, out, nf);
+  const double* ri[6];
+  double* ro[6];
+  as_real(in, ri, nf);
+  for (int i = 0; i < nf; ++i) ro[i] = reinterpret_cast<double*>(out[i]);
+  return synth_t<double, double, double>(c, c.costab, 1, c.d.Ph, c.d.Py, ri, ro, nf);
 }
 
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return synth(c, 2, c.d.Pn, c.d.Ny, in, out, nf);
+  const double* ri[6];
+  double* ro[6];
+  as_real(in, ri, nf);
+  for (int i = 0; i < nf; ++i) ro[i] = reinterpret_cast<double*>(out[i]);
+  return synth_t<double, double, double>(c, c.costab, 2, c.d.Pn, c.d.Ny, ri, ro, nf);
+}
+
+// Mixed form: FP64 coefficients -> FP32 values on the padded grid, FP32 matrix cores.
+hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const* out, int nf) {
+  const double* ri[6];
+  float* ro[6];
+  as_real(in, ri, nf);
+  for (int i = 0; i < nf; ++i) ro[i] = reinterpret_cast<float*>(out[i]);
+  return synth_t<double, float, float>(c, c.costabf, 1, c.d.Ph, c.d.Py, ri, ro, nf);
 }
 
 // ----------------------------------------------------------------- analysis
 // coefficient k = 2j+m: (1/(2N c_k)) sum_p w_p cos(pi k p/2N) (x_p +- x_{2N-p})
 // over the folded points p = 0..N (PF:1686-1700: /2N, c_0 = 2, w_0 = 1, else 2).
+template <class TI, class TO, class TCc>
 struct AnalP {
+  using TC = TCc;
   static constexpr int NACC = 2, NB = 2;
   static constexpr int bsel(int m) { return m; }
-  const double* costab;
+  const TC* costab;
   int n4;
   MfmaShape sh;  // M = ne, K = Ph
-  const double* inf[6];
-  double* outf[6];
-  const double* in;
-  double* out;
+  const TI* inf[6];
+  TO* outf[6];
+  const TI* in;
+  TO* out;
   int P, N;
   long ncold;
   double inv2n;
@@ -107,33 +133,34 @@ struct AnalP {
     return (int)(((long)(2 * row + m) * k0) % n4_);
   }
   __device__ int astep(int m, int row, int n4_) const { return (int)((4L * (2 * row + m)) % n4_); }
-  __device__ void loadB(int p, long col, double* v) const {
-    const double x = in[p * ncold + col];
+  __device__ void loadB(int p, long col, TC* v) const {
+    const TC x = (TC)in[p * ncold + col];
     const int q = P - 1 - p;
-    const double w = (p == 0) ? 1.0 : 2.0;
+    const TC w = (p == 0) ? (TC)1 : (TC)2;
     if (q == p) {
       v[0] = w * x;
-      v[1] = 0.0;
+      v[1] = 0;
     } else {
-      const double y = in[q * ncold + col];
+      const TC y = (TC)in[q * ncold + col];
       v[0] = w * (x + y);
       v[1] = w * (x - y);
     }
   }
-  __device__ void store(int r, long col, const double* acc) const {
+  __device__ void store(int r, long col, const TC* acc) const {
     const int k = 2 * r;
-    out[k * ncold + col] = acc[0] * (k == 0 ? 0.5 * inv2n : inv2n);
-    if (k + 1 <= N) out[(k + 1) * ncold + col] = acc[1] * inv2n;
+    out[k * ncold + col] = (TO)((double)acc[0] * (k == 0 ? 0.5 * inv2n : inv2n));
+    if (k + 1 <= N) out[(k + 1) * ncold + col] = (TO)((double)acc[1] * inv2n);
   }
 };
 
-hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  AnalP pb{};
-  pb.costab = c.costab;
+template <class TI, class TC>
+static hipError_t analysis_t(Ctx& c, const TC* costab, const TI* const* in, double2* const* out, int nf) {
+  AnalP<TI, double, TC> pb{};
+  pb.costab = costab;
   pb.n4 = 4 * c.d.N;
   pb.sh = mfma_shape(c.d.ne, c.d.Ph);
   for (int i = 0; i < nf; ++i) {
-    pb.inf[i] = reinterpret_cast<const double*>(in[i]);
+    pb.inf[i] = in[i];
     pb.outf[i] = reinterpret_cast<double*>(out[i]);
   }
   pb.P = c.d.Py;
@@ -141,6 +168,19 @@ hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out
   pb.ncold = 2 * c.d.ncol;
   pb.inv2n = 1.0 / (2.0 * c.d.N);
   return run_mfma_cos(c.stream, pb, nf);
+}
+
+hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
+  const double* ri[6];
+  as_real(in, ri, nf);
+  return analysis_t<double, double>(c, c.costab, ri, out, nf);
+}
+
+// Mixed form: FP32 values on the padded grid -> FP64 coefficients, FP32 matrix cores.
+hipError_t launch_analysis32(Ctx& c, const float2* const* in, double2* const* out, int nf) {
+  const float* ri[6];
+  for (int i = 0; i < nf; ++i) ri[i] = reinterpret_cast<const float*>(in[i]);
+  return analysis_t<float, float>(c, c.costabf, ri, out, nf);
 }
 
 // ----------------------------------------------------------------------- rhs

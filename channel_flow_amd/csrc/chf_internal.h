@@ -80,6 +80,8 @@ struct Ctx {
   // nonlinear work
   double2* Y[6] = {};     // [Py][Mz][Mx]   wall-normal points, Fourier modes
   double2* Z[6] = {};     // [Py][Lz][Mx]   wall-normal + span points, x modes
+  float2* Yf[6] = {};     // mixed form: FP32 copies of Y and Z
+  float2* Zf[6] = {};
   double2* Lc[3] = {};    // [Ny][Mz][Mx]   Lamb vector coefficients
   double2 *hvc = nullptr, *hgc = nullptr;  // h coefficients
   // solve work
@@ -90,6 +92,8 @@ struct Ctx {
          *t4 = nullptr;
   ChebOp op_rv, op_ro;     // packed explicit-operator fragments (rhs)
   double* costab = nullptr;  // cos(pi n / 2N), n < 4N
+  float* costabf = nullptr;  // the same rounded to FP32 (mixed form)
+  float2 *twxf = nullptr, *twzf = nullptr;  // FP32 twiddles (mixed form)
   double* invV = nullptr;  // [ngroups][ne^2 + no^2] transposed inverses (v)
   double* invW = nullptr;  // [ngroups][ne^2 + no^2] (omega_y; group 0 = mean mode)
   double2* twx = nullptr;  // exp(-2 pi i m/Lx), m < Lx
@@ -126,6 +130,11 @@ enum KernelId {
 extern const char* const kKernelNames[K_COUNT];
 
 // ---- kernel launchers (each returns hipGetLastError()) ----
+bool reg_fft_length(int L);
+hipError_t launch_zfft32(Ctx& c, const float2* const* in, float2* const* out, int nf, bool inv);
+hipError_t launch_xpass32(Ctx& c);
+hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const* out, int nf);
+hipError_t launch_analysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_zfft_inv(Ctx& c, double2* const* in, double2* const* out, int nf);
 hipError_t launch_xpass(Ctx& c);
 hipError_t launch_zfft_fwd(Ctx& c, double2* const* in, double2* const* out, int nf);

@@ -185,7 +185,8 @@ static int pick_tx(int Mx, int Lz) {
   return tx;
 }
 
-static bool try_zfft_reg(Ctx& c, const ZArgs& a, bool inv, int nf, hipError_t& err);
+static hipError_t zfft_reg64(Ctx& c, double2* const* in, double2* const* out, int nf, bool inv);
+static bool use_zreg(const Ctx& c);
 
 static hipError_t launch_z(Ctx& c, double2* const* in, double2* const* out, int nf, bool inv) {
   const Dims& d = c.d;
@@ -202,8 +203,7 @@ static hipError_t launch_z(Ctx& c, double2* const* in, double2* const* out, int 
   a.out_plane = (long)(inv ? d.Lz : d.Mz) * d.Mx;
   a.pl = make_plan(d.Lz);
   a.tw = c.twz;
-  hipError_t rerr = hipSuccess;
-  if (try_zfft_reg(c, a, inv, nf, rerr)) return rerr;
+  if (use_zreg(c)) return zfft_reg64(c, in, out, nf, inv);
   dim3 grid(d.Mx / a.TX, d.Py, nf);
   size_t shm = 2ull * a.TX * d.Lz * sizeof(double2);
   if (inv)
@@ -302,7 +302,8 @@ __global__ __launch_bounds__(256) void k_xpass(XArgs a) {
 
 // ------------------------------------------- register-resident variants
 // Line length L = N1 * N2 handled by a group of G = max(N1, N2) threads with
-// one padded LDS exchange per transform (see fftreg.h).
+// one padded LDS exchange per transform (see fftreg.h).  V is double2 (FP64
+// form) or float2 (physical-space FP32 of the mixed form).
 
 template <int N1, int N2>
 struct Geo {
@@ -313,40 +314,61 @@ struct Geo {
   static constexpr int BUF = (BUF_A > BUF_B ? BUF_A : BUF_B) > L ? (BUF_A > BUF_B ? BUF_A : BUF_B) : L;
 };
 
+template <class V>
+struct RegX {
+  const V* in[6];  // u, v, w, omega_x, omega_y, omega_z  [Py][Lz][Mx]
+  V* out[3];       // L_x, L_y, L_z                        [Py][Lz][Mx]
+  int Mx;
+  long nlines;     // Py * Lz
+  double re_p, scale;
+  const V* tw;     // e^{-2 pi i m / Lx}
+};
+
+template <class V>
+struct RegZ {
+  const V* in[6];
+  V* out[6];
+  int Mx, Mz;
+  long in_plane, out_plane;
+  const V* tw;     // e^{-2 pi i m / Lz}
+};
+
 // Streamwise pass: one group per (y point, span point) line.
-template <int N1, int N2, int NG>
-__global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(XArgs a) {
+template <int N1, int N2, int NG, class V>
+__global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(RegX<V> a) {
   using Gm = Geo<N1, N2>;
+  using T = decltype(V{}.x);
   constexpr int L = Gm::L, G = Gm::G;
-  extern __shared__ double2 sm[];
-  double2* tw = sm;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  V* tw = sm;
   const int g = threadIdx.x / G, t = threadIdx.x - g * G;
-  double2* buf = sm + L + g * Gm::BUF;
+  V* buf = sm + L + g * Gm::BUF;
   const long line = (long)blockIdx.x * NG + g;
   const bool valid = line < a.nlines;
   const int Mx = a.Mx;
   for (int i = threadIdx.x; i < L; i += NG * G) tw[i] = a.tw[i];
   __syncthreads();
-  double2 r[3][N1];
+  V r[3][N1];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (t < N1) {
-      double2 x[N2];
-      const double2* f1 = a.in[2 * q] + line * Mx;
-      const double2* f2 = a.in[2 * q + 1] + line * Mx;
+      V x[N2];
+      const V* f1 = a.in[2 * q] + line * Mx;
+      const V* f2 = a.in[2 * q + 1] + line * Mx;
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) {
         const int m = n2 * N1 + t;
-        double2 val = make_double2(0.0, 0.0);
+        V val = C2<T>::make(0, 0);
         if (valid) {
           if (m == 0) {
-            val = make_double2(f1[0].x, f2[0].x);          // Re of the kx = 0 entry (PF:1116-1118)
+            val = C2<T>::make(f1[0].x, f2[0].x);            // Re of the kx = 0 entry (PF:1116-1118)
           } else if (m < Mx) {
-            const double2 A = f1[m], B = f2[m];
-            val = make_double2(A.x - B.y, A.y + B.x);       // F1 + i F2
+            const V A = f1[m], B = f2[m];
+            val = C2<T>::make(A.x - B.y, A.y + B.x);         // F1 + i F2
           } else if (m > L - Mx) {
-            const double2 A = f1[L - m], B = f2[L - m];
-            val = make_double2(A.x + B.y, B.x - A.y);       // conj(F1) + i conj(F2)
+            const V A = f1[L - m], B = f2[L - m];
+            val = C2<T>::make(A.x + B.y, B.x - A.y);         // conj(F1) + i conj(F2)
           }
         }
         x[n2] = val;
@@ -358,25 +380,27 @@ __global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(XArgs a) {
     __syncthreads();
   }
   // Lamb vector at the physical points x = k1 N2 + t: (L_x + i L_y) into r[0], (L_z + 0i) into r[1]
+  const T rep = (T)a.re_p;
   if (t < N2) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
-      const double uu = r[0][k1].x, vv = r[0][k1].y, ww = r[1][k1].x, oxx = r[1][k1].y;
-      const double oyy = r[2][k1].x, ozz = r[2][k1].y;
-      const double lx = (oyy * ww - a.re_p) - ozz * vv;
-      const double ly = ozz * uu - oxx * ww;
-      const double lz = oxx * vv - oyy * uu;
-      r[0][k1] = make_double2(lx, ly);
-      r[1][k1] = make_double2(lz, 0.0);
+      const T uu = r[0][k1].x, vv = r[0][k1].y, ww = r[1][k1].x, oxx = r[1][k1].y;
+      const T oyy = r[2][k1].x, ozz = r[2][k1].y;
+      const T lx = (oyy * ww - rep) - ozz * vv;
+      const T ly = ozz * uu - oxx * ww;
+      const T lz = oxx * vv - oyy * uu;
+      r[0][k1] = C2<T>::make(lx, ly);
+      r[1][k1] = C2<T>::make(lz, 0);
     }
   }
-  const double sc = a.scale;
+  const T sc = (T)a.scale;
+  const T half = (T)0.5;
   // forward transforms with the transposed split: thread t holds x[n2' N2 + t], n2' < N1
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     if (t < N2) fft_stepA<N2, N1, -1>(r[q], buf, tw, t);
     __syncthreads();
-    double2 b[N2];
+    V b[N2];
     if (t < N1) fft_stepB<N2, N1, -1>(b, buf, t);     // X[k1' N1 + t], k1' < N2
     __syncthreads();
     if (q == 0) {
@@ -388,10 +412,10 @@ __global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(XArgs a) {
       __syncthreads();
       if (valid) {
         for (int m = t; m < Mx; m += G) {
-          const double2 Gp = buf[m];
-          const double2 Gc = buf[m == 0 ? 0 : L - m];
-          a.out[0][line * Mx + m] = make_double2(0.5 * (Gp.x + Gc.x) * sc, 0.5 * (Gp.y - Gc.y) * sc);
-          a.out[1][line * Mx + m] = make_double2(0.5 * (Gp.y + Gc.y) * sc, -0.5 * (Gp.x - Gc.x) * sc);
+          const V Gp = buf[m];
+          const V Gc = buf[m == 0 ? 0 : L - m];
+          a.out[0][line * Mx + m] = C2<T>::make(half * (Gp.x + Gc.x) * sc, half * (Gp.y - Gc.y) * sc);
+          a.out[1][line * Mx + m] = C2<T>::make(half * (Gp.y + Gc.y) * sc, -half * (Gp.x - Gc.x) * sc);
         }
       }
       __syncthreads();
@@ -399,7 +423,7 @@ __global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(XArgs a) {
 #pragma unroll
       for (int k = 0; k < N2; ++k) {
         const int m = k * N1 + t;
-        if (m < Mx) a.out[2][line * Mx + m] = make_double2(b[k].x * sc, b[k].y * sc);
+        if (m < Mx) a.out[2][line * Mx + m] = C2<T>::make(b[k].x * sc, b[k].y * sc);
       }
     }
   }
@@ -407,23 +431,25 @@ __global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(XArgs a) {
 
 // Span passes: a group handles TX span lines (consecutive kx) of one plane;
 // thread (t, tx) = position t of line tx, tx fastest for coalesced access.
-template <int N1, int N2, int TX, bool INV>
-__global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(ZArgs a) {
+template <int N1, int N2, int TX, bool INV, class V>
+__global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(RegZ<V> a) {
   using Gm = Geo<N1, N2>;
+  using T = decltype(V{}.x);
   constexpr int L = Gm::L, G = Gm::G;
-  extern __shared__ double2 sm[];
-  double2* tw = sm;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  V* tw = sm;
   const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
-  double2* buf = sm + L + tx * Gm::BUF_A;
+  V* buf = sm + L + tx * Gm::BUF_A;
   const int Mz = a.Mz, h = Mz / 2;
   const int kx = blockIdx.x * TX + tx;
   const int p = blockIdx.y;
-  const double2* in = a.in[blockIdx.z] + (long)p * a.in_plane;
-  double2* out = a.out[blockIdx.z] + (long)p * a.out_plane;
+  const V* in = a.in[blockIdx.z] + (long)p * a.in_plane;
+  V* out = a.out[blockIdx.z] + (long)p * a.out_plane;
   for (int i = threadIdx.x; i < L; i += TX * G) tw[i] = a.tw[i];
   __syncthreads();
   if (t < N1) {
-    double2 x[N2];
+    V x[N2];
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) {
       const int m = n2 * N1 + t;
@@ -431,7 +457,7 @@ __global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(ZArgs a) {
         int kz = -1;
         if (m < h) kz = m;                       // Nyquist slot Mz/2 stays zero (expand, PF:730)
         else if (m > L - h) kz = m - L + Mz;
-        x[n2] = kz >= 0 ? in[(long)kz * a.Mx + kx] : make_double2(0.0, 0.0);
+        x[n2] = kz >= 0 ? in[(long)kz * a.Mx + kx] : C2<T>::make(0, 0);
       } else {
         x[n2] = in[(long)m * a.Mx + kx];
       }
@@ -440,7 +466,7 @@ __global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(ZArgs a) {
   }
   __syncthreads();
   if (t < N2) {
-    double2 b[N1];
+    V b[N1];
     fft_stepB<N1, N2, INV ? +1 : -1>(b, buf, t);
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
@@ -457,78 +483,115 @@ __global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(ZArgs a) {
   }
 }
 
-// (N1, N2) splits with register-resident kernels; 0 = use the LDS Stockham kernel.
-static void reg_split(int L, int& n1, int& n2) {
-  n1 = n2 = 0;
+bool reg_fft_length(int L) {
   switch (L) {
-    case 12: n1 = 4; n2 = 3; break;
-    case 24: n1 = 4; n2 = 6; break;
-    case 48: n1 = 6; n2 = 8; break;
-    case 96: n1 = 8; n2 = 12; break;
-    case 144: n1 = 12; n2 = 12; break;
-    case 192: n1 = 12; n2 = 16; break;
-    case 384: n1 = 16; n2 = 24; break;
-    default: break;
+    case 12: case 24: case 48: case 96: case 144: case 192: case 384: return true;
+    default: return false;
   }
 }
 
-template <int N1, int N2>
-static hipError_t launch_xpass_reg(Ctx& c, const XArgs& a) {
+template <int N1, int N2, class V>
+static hipError_t run_xpass_reg(Ctx& c, const RegX<V>& a) {
   using Gm = Geo<N1, N2>;
   constexpr int NG = 256 / Gm::G;
   const long blocks = (a.nlines + NG - 1) / NG;
-  const size_t shm = sizeof(double2) * (Gm::L + (size_t)NG * Gm::BUF);
+  const size_t shm = sizeof(V) * (Gm::L + (size_t)NG * Gm::BUF);
   if (shm > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_xpass_reg<N1, N2, NG>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_xpass_reg<N1, N2, NG, V>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((k_xpass_reg<N1, N2, NG>), dim3((unsigned)blocks), dim3(NG * Gm::G), shm, c.stream, a);
+  hipLaunchKernelGGL((k_xpass_reg<N1, N2, NG, V>), dim3((unsigned)blocks), dim3(NG * Gm::G), shm, c.stream, a);
   return hipGetLastError();
 }
 
-template <int N1, int N2>
-static hipError_t launch_zfft_reg(Ctx& c, const ZArgs& a, bool inv, int py, int nf) {
+template <class V>
+static hipError_t xpass_reg(Ctx& c, const RegX<V>& a) {
+  switch (c.d.Lx) {
+    case 12: return run_xpass_reg<4, 3>(c, a);
+    case 24: return run_xpass_reg<4, 6>(c, a);
+    case 48: return run_xpass_reg<6, 8>(c, a);
+    case 96: return run_xpass_reg<8, 12>(c, a);
+    case 144: return run_xpass_reg<12, 12>(c, a);
+    case 192: return run_xpass_reg<12, 16>(c, a);
+    case 384: return run_xpass_reg<16, 24>(c, a);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N1, int N2, class V>
+static hipError_t run_zfft_reg(Ctx& c, const RegZ<V>& a, bool inv, int nf) {
   using Gm = Geo<N1, N2>;
   constexpr int TX = (256 / Gm::G) >= 16 ? 16 : ((256 / Gm::G) >= 8 ? 8 : 4);
   if (c.d.Mx % TX) return hipErrorInvalidValue;
-  const size_t shm = sizeof(double2) * (Gm::L + (size_t)TX * Gm::BUF_A);
-  dim3 grid(c.d.Mx / TX, py, nf);
-  if (inv) {
-    if (shm > 65536) {
-      hipError_t e = hipFuncSetAttribute((const void*)k_zfft_reg<N1, N2, TX, true>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-      if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_zfft_reg<N1, N2, TX, true>), grid, dim3(TX * Gm::G), shm, c.stream, a);
-  } else {
-    if (shm > 65536) {
-      hipError_t e = hipFuncSetAttribute((const void*)k_zfft_reg<N1, N2, TX, false>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-      if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_zfft_reg<N1, N2, TX, false>), grid, dim3(TX * Gm::G), shm, c.stream, a);
+  const size_t shm = sizeof(V) * (Gm::L + (size_t)TX * Gm::BUF_A);
+  dim3 grid(c.d.Mx / TX, c.d.Py, nf);
+  const void* fn = inv ? (const void*)k_zfft_reg<N1, N2, TX, true, V> : (const void*)k_zfft_reg<N1, N2, TX, false, V>;
+  if (shm > 65536) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
   }
+  if (inv)
+    hipLaunchKernelGGL((k_zfft_reg<N1, N2, TX, true, V>), grid, dim3(TX * Gm::G), shm, c.stream, a);
+  else
+    hipLaunchKernelGGL((k_zfft_reg<N1, N2, TX, false, V>), grid, dim3(TX * Gm::G), shm, c.stream, a);
   return hipGetLastError();
 }
 
-// Dispatches a span pass to the register kernel when the length has a split
-// (and Mx is a multiple of the kernel's line tile), returns false otherwise.
-static bool try_zfft_reg(Ctx& c, const ZArgs& a, bool inv, int nf, hipError_t& err) {
-  int n1, n2;
-  reg_split(c.d.Lz, n1, n2);
-  if (!n1 || c.d.Mx % 16) return false;
+// Span pass through the register kernel; hipErrorInvalidValue when the length
+// has no split or Mx is not a multiple of the kernel's line tile (16).
+template <class V>
+static hipError_t zfft_reg(Ctx& c, const RegZ<V>& a, bool inv, int nf) {
+  if (c.d.Mx % 16) return hipErrorInvalidValue;
   switch (c.d.Lz) {
-    case 12: err = launch_zfft_reg<4, 3>(c, a, inv, c.d.Py, nf); break;
-    case 24: err = launch_zfft_reg<4, 6>(c, a, inv, c.d.Py, nf); break;
-    case 48: err = launch_zfft_reg<6, 8>(c, a, inv, c.d.Py, nf); break;
-    case 96: err = launch_zfft_reg<8, 12>(c, a, inv, c.d.Py, nf); break;
-    case 144: err = launch_zfft_reg<12, 12>(c, a, inv, c.d.Py, nf); break;
-    case 192: err = launch_zfft_reg<12, 16>(c, a, inv, c.d.Py, nf); break;
-    case 384: err = launch_zfft_reg<16, 24>(c, a, inv, c.d.Py, nf); break;
-    default: return false;
+    case 12: return run_zfft_reg<4, 3>(c, a, inv, nf);
+    case 24: return run_zfft_reg<4, 6>(c, a, inv, nf);
+    case 48: return run_zfft_reg<6, 8>(c, a, inv, nf);
+    case 96: return run_zfft_reg<8, 12>(c, a, inv, nf);
+    case 144: return run_zfft_reg<12, 12>(c, a, inv, nf);
+    case 192: return run_zfft_reg<12, 16>(c, a, inv, nf);
+    case 384: return run_zfft_reg<16, 24>(c, a, inv, nf);
+    default: return hipErrorInvalidValue;
   }
-  return true;
+}
+
+template <class V>
+static RegZ<V> regz_args(Ctx& c, const V* const* in, V* const* out, int nf, bool inv, const V* tw) {
+  RegZ<V> a{};
+  for (int f = 0; f < nf; ++f) {
+    a.in[f] = in[f];
+    a.out[f] = out[f];
+  }
+  a.Mx = c.d.Mx;
+  a.Mz = c.d.Mz;
+  a.in_plane = (long)(inv ? c.d.Mz : c.d.Lz) * c.d.Mx;
+  a.out_plane = (long)(inv ? c.d.Lz : c.d.Mz) * c.d.Mx;
+  a.tw = tw;
+  return a;
+}
+
+static bool use_zreg(const Ctx& c) { return reg_fft_length(c.d.Lz) && c.d.Mx % 16 == 0; }
+
+static hipError_t zfft_reg64(Ctx& c, double2* const* in, double2* const* out, int nf, bool inv) {
+  const double2* cin[6];
+  for (int f = 0; f < nf; ++f) cin[f] = in[f];
+  return zfft_reg(c, regz_args<double2>(c, cin, out, nf, inv, c.twz), inv, nf);
+}
+
+hipError_t launch_zfft32(Ctx& c, const float2* const* in, float2* const* out, int nf, bool inv) {
+  return zfft_reg(c, regz_args<float2>(c, in, out, nf, inv, c.twzf), inv, nf);
+}
+
+hipError_t launch_xpass32(Ctx& c) {
+  RegX<float2> a{};
+  for (int f = 0; f < 6; ++f) a.in[f] = c.Zf[f];
+  for (int f = 0; f < 3; ++f) a.out[f] = c.Zf[f];
+  a.Mx = c.d.Mx;
+  a.nlines = (long)c.d.Py * c.d.Lz;
+  a.re_p = c.prm.re_p;
+  a.scale = 1.0 / ((double)c.d.Lx * (double)c.d.Lz);
+  a.tw = c.twxf;
+  return xpass_reg(c, a);
 }
 
 hipError_t launch_xpass(Ctx& c) {
@@ -548,15 +611,16 @@ hipError_t launch_xpass(Ctx& c) {
   a.scale = 1.0 / ((double)d.Lx * (double)d.Lz);
   a.pl = make_plan(d.Lx);
   a.tw = c.twx;
-  switch (d.Lx) {
-    case 12: return launch_xpass_reg<4, 3>(c, a);
-    case 24: return launch_xpass_reg<4, 6>(c, a);
-    case 48: return launch_xpass_reg<6, 8>(c, a);
-    case 96: return launch_xpass_reg<8, 12>(c, a);
-    case 144: return launch_xpass_reg<12, 12>(c, a);
-    case 192: return launch_xpass_reg<12, 16>(c, a);
-    case 384: return launch_xpass_reg<16, 24>(c, a);
-    default: break;  // longer lines: LDS Stockham kernel below
+  if (reg_fft_length(d.Lx)) {
+    RegX<double2> r{};
+    for (int f = 0; f < 6; ++f) r.in[f] = c.Z[f];
+    for (int f = 0; f < 3; ++f) r.out[f] = c.Z[f];
+    r.Mx = d.Mx;
+    r.nlines = a.nlines;
+    r.re_p = a.re_p;
+    r.scale = a.scale;
+    r.tw = c.twx;
+    return xpass_reg(c, r);
   }
   dim3 grid((unsigned)((a.nlines + lpb - 1) / lpb));
   size_t shm = 2ull * 3 * lpb * d.Lx * sizeof(double2);

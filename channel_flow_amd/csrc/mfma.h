@@ -15,6 +15,29 @@
 namespace chf {
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// Matrix-core traits per compute type.  Operand layouts (both types, probed):
+// A lane l = A[l%16][l/16], B lane l = B[l/16][l%16]; the 16x16 result rows
+// differ: FP64 lane l element e = D[l/16 + 4e][l%16], FP32 = D[4(l/16) + e][l%16].
+template <class TC>
+struct Mf;
+template <>
+struct Mf<double> {
+  using acc = dvec4;
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int e) { return (lane >> 4) + 4 * e; }
+};
+template <>
+struct Mf<float> {
+  using acc = fvec4;
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
+};
 
 template <class P, int RBW, int CB>
 __global__ __launch_bounds__(256) void k_mfma(P pb) {
@@ -106,11 +129,14 @@ __global__ __launch_bounds__(256) void k_mfma(P pb) {
 // current one feeds the matrix cores.
 template <class P, int RBW, int CB>
 __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
+  using TC = typename P::TC;
+  using MF = Mf<TC>;
   constexpr int NA = P::NACC, NB = P::NB, KC = 16, BN = 16 * CB;
   constexpr int SB = (BN % 32 == 0) ? BN + 16 : BN;
   constexpr int PER = (KC * BN + 255) / 256;  // tile elements per thread
-  __shared__ double sB[NB][KC][SB];
-  extern __shared__ double stab[];           // 4N cosines
+  __shared__ TC sB[NB][KC][SB];
+  extern __shared__ __align__(16) unsigned char stabraw[];
+  TC* stab = reinterpret_cast<TC*>(stabraw);  // 4N cosines
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   P p = pb;
   p.select(blockIdx.z);
@@ -129,15 +155,15 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
       idx[m][r] = p.aindex(m, row, k0, n4);
       stp[m][r] = p.astep(m, row, n4);  // index advance for k -> k + 4
     }
-  dvec4 acc[NA][RBW][CB];
+  typename MF::acc acc[NA][RBW][CB];
 #pragma unroll
   for (int m = 0; m < NA; ++m)
 #pragma unroll
     for (int r = 0; r < RBW; ++r)
 #pragma unroll
-      for (int c = 0; c < CB; ++c) acc[m][r][c] = dvec4{0.0, 0.0, 0.0, 0.0};
+      for (int c = 0; c < CB; ++c) acc[m][r][c] = typename MF::acc{0, 0, 0, 0};
   const int nkc = (KB * 4 + KC - 1) / KC;
-  double nxt[PER][NB];
+  TC nxt[PER][NB];
   auto fetch = [&](int kc) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -149,7 +175,7 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
         p.loadB(k, col, nxt[q]);
       } else {
 #pragma unroll
-        for (int s = 0; s < NB; ++s) nxt[q][s] = 0.0;
+        for (int s = 0; s < NB; ++s) nxt[q][s] = 0;
       }
     }
   };
@@ -170,7 +196,7 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
     for (int kb = 0; kb < KC / 4; ++kb) {
       const int kg = kc * (KC / 4) + kb;
       if (kg < KB) {
-        double bv[NB][CB];
+        TC bv[NB][CB];
 #pragma unroll
         for (int s = 0; s < NB; ++s)
 #pragma unroll
@@ -181,12 +207,12 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
           if (rb < MB) {
 #pragma unroll
             for (int m = 0; m < NA; ++m) {
-              const double a = stab[idx[m][r]];
+              const TC a = stab[idx[m][r]];
               int t = idx[m][r] + stp[m][r];
               idx[m][r] = t >= n4 ? t - n4 : t;
 #pragma unroll
               for (int c = 0; c < CB; ++c)
-                acc[m][r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[P::bsel(m)][c], acc[m][r][c], 0, 0, 0);
+                acc[m][r][c] = MF::mma(a, bv[P::bsel(m)][c], acc[m][r][c]);
             }
           }
         }
@@ -204,10 +230,9 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
       if (col >= p.ncold) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        // v_mfma_f64_16x16x4f64 result: lane l, element e holds D[l/16 + 4e][l%16]
-        const int row = rb * 16 + (lane >> 4) + 4 * e;
+        const int row = rb * 16 + MF::row(lane, e);
         if (row >= p.sh.M) continue;
-        double v[NA];
+        TC v[NA];
 #pragma unroll
         for (int m = 0; m < NA; ++m) v[m] = acc[m][r][c][e];
         p.store(row, col, v);
@@ -223,7 +248,7 @@ hipError_t run_mfma_cos_rbw(hipStream_t st, const P& p, int nz) {
   constexpr int CB = budget >= 4 ? 4 : (budget >= 2 ? 2 : 1);
   const long tiles = (p.ncold + 16 * CB - 1) / (16 * CB);
   hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB>), dim3((unsigned)tiles, 1, nz), dim3(256),
-                     sizeof(double) * p.n4, st, p);
+                     sizeof(typename P::TC) * p.n4, st, p);
   return hipGetLastError();
 }
 

@@ -151,3 +151,35 @@ def test_laminar_fixed_point_gpu(require_gpu):
         got = ch.get_state()
     assert rel_err(got["au0"], au0) < 1e-13
     assert np.max(np.abs(got["vnp1"])) == 0.0
+
+
+def test_mixed_form_against_fp64_and_oracle(require_gpu):
+    """Mixed form (FP32 transforms/products, FP64 solves/state) on config 1.
+
+    Tolerances: the Lamb vector and h_g carry FP32 rounding (~1e-7 relative);
+    h_v = k^2 L_y + i alpha D L_x + i beta D L_z differentiates the Lamb
+    coefficients, which multiplies the FP32 noise floor of the highest
+    Chebyshev degrees by up to ~N^2 (max-norm ~1e-3 at N = 32).  The FP64
+    solve damps those degrees, so v and omega_y after 10 steps agree with the
+    FP64 form to ~1e-7 and the mean flow to ~1e-13."""
+    cfg = ic.CONFIGS["cfg1_32x33x32"]
+    p = params_of(cfg)
+    v, oy, au0, w0 = ic_of(cfg)
+    ref = O.initial_presolve_state(v, oy, au0, w0, p)
+    with make_channel(cfg, "mixed") as mx, make_channel(cfg, "fp64") as fp:
+        mx.initial(v, oy, au0, w0)
+        fp.initial(v, oy, au0, w0)
+        e0 = state_errors(mx.get_state(), ref)
+        report("mixed initial vs oracle", e0)
+        mx.step(10)
+        fp.step(10)
+        gm, gf = mx.get_state(), fp.get_state()
+    e1 = state_errors(gm, gf)
+    report("mixed vs fp64 after 10 steps", e1)
+    assert e0["rhsvn"] < 5e-3, e0["rhsvn"]
+    for k in ("rhsomegayn", "clamb1n", "clamb3n"):
+        assert e0[k] < 1e-5, (k, e0[k])
+    for k in ("vnp1", "omegaynp1"):
+        assert e1[k] < 1e-6, (k, e1[k])
+    for k in ("au0", "w0"):
+        assert e1[k] < 1e-10, (k, e1[k])
