@@ -26,6 +26,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, GB/s
 FP64_PEAK_TFS = 78.6         # MI355X dense FP64 (matrix and vector), TFLOP/s
+FP32_PEAK_TFS = 157.3        # MI355X dense FP32 matrix, TFLOP/s
 
 
 def dims(c):
@@ -37,17 +38,20 @@ def dims(c):
     return d
 
 
-def kernel_work(d, word=16):
+def kernel_work(d, precision="fp64"):
     """Algorithmic HBM bytes and flops per launch of each kernel class.
 
     Each stage reads its inputs once and writes its outputs once at the
-    storage precision (complex = `word` bytes); flops count the useful
-    multiply-adds of the parity-split contractions (2 flops each).
+    storage precision: spectral fields and the solve are FP64 (16 B per
+    complex value); the wall-normal/span point arrays of the nonlinear term are
+    FP64 in the FP64 form and FP32 (8 B) in the mixed form.  Flops count the
+    useful multiply-adds (2 flops each) of the parity-split contractions.
     """
-    S = d["Ny"] * d["ncol"] * word           # spectral field
-    Y = d["Py"] * d["ncol"] * word           # wall-normal points, Fourier modes
-    Z = d["Py"] * d["Lz"] * d["Mx"] * word   # wall-normal + span points
-    col = 2 * d["ncol"]                      # real columns
+    w = 16 if precision == "fp64" else 8
+    S = d["Ny"] * d["ncol"] * 16            # spectral field (FP64)
+    Y = d["Py"] * d["ncol"] * w             # wall-normal points, Fourier modes
+    Z = d["Py"] * d["Lz"] * d["Mx"] * w     # wall-normal + span points
+    col = 2 * d["ncol"]                     # real columns
     ne, no, Ny, Ph, Pn = d["ne"], d["no"], d["Ny"], d["Ph"], d["Pn"]
     rv_rows = (ne - 2) + (no - 2)
     ro_rows = (ne - 1) + (no - 1)
@@ -61,21 +65,26 @@ def kernel_work(d, word=16):
         "hassemble": (5 * S, 0.0),
         "synth_points": (4 * S, 2.0 * Pn * Ny * col * 2),
         "mean_points": (4 * Ny * 16, 0.0),
-        "rhs_v": (S + 2 * (Ny - 4) / Ny * S + rv_rows / Ny * S, 3 * 2.0 * (rv_rows * ne) * col),
-        "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S, 2 * 2.0 * (ro_rows * ne) * col),
+        "rhs_prep": (4 * S, 0.0),
+        "rhs_v": (S + 2 * (Ny - 4) / Ny * S + rv_rows / Ny * S,
+                  2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
+        "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S,
+                  2.0 * ((ne - 1) * ne + (no - 1) * no) * col),
         "solve": (inv_bytes + (rv_rows + ro_rows) / Ny * S + 6 * S, 0.0),
     }
 
 
-def roofline_of(name, avg_ms, work):
+def roofline_of(name, avg_ms, work, precision="fp64"):
     nbytes, flops = work[name]
     sec = avg_ms * 1e-3
     gbs = nbytes / sec / 1e9
     tfs = flops / sec / 1e12 if flops else 0.0
+    fp32_mm = precision == "mixed" and name in ("synth_padded", "analysis")
+    peak_tf = FP32_PEAK_TFS if fp32_mm else FP64_PEAK_TFS
     # bound = the resource whose time at peak is larger
-    if flops and flops / (FP64_PEAK_TFS * 1e12) > nbytes / (HBM_PEAK_GBS * 1e9):
-        return dict(kernel=name, bound="mfma", achieved=round(tfs, 3), peak=FP64_PEAK_TFS,
-                    unit="TFLOP/s", frac=round(tfs / FP64_PEAK_TFS, 4), avg_ms=round(avg_ms, 4),
+    if flops and flops / (peak_tf * 1e12) > nbytes / (HBM_PEAK_GBS * 1e9):
+        return dict(kernel=name, bound="mfma", achieved=round(tfs, 3), peak=peak_tf,
+                    unit="TFLOP/s", frac=round(tfs / peak_tf, 4), avg_ms=round(avg_ms, 4),
                     algorithmic_bytes=int(nbytes), flops=flops, achieved_GBs=round(gbs, 1))
     return dict(kernel=name, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), avg_ms=round(avg_ms, 4),
@@ -131,13 +140,80 @@ def cpu_baseline(cfg, seconds_budget=30.0):
                        f"-> {sec_per_step:.1f} s/step")}
 
 
+def run_form(cfg, ic_arrays, precision, steps, warmup, total, log):
+    """Set-up, warm-up, a timed region with per-kernel HIP events, then
+    untimed steps up to `total` for the error report."""
+    from channel_flow_amd.channel import Channel
+    v, oy, au0, w0 = ic_arrays
+    t0 = time.perf_counter()
+    ch = Channel(cfg["nnx"], cfg["nny"], cfg["nnz"], cfg["aphi"], cfg["beta"], cfg["dt"], cfg["re"],
+                 cfg["re_p"], precision=precision)
+    t_setup = time.perf_counter() - t0
+    log(f"{precision}: setup {t_setup:.2f}s, device bytes {ch.device_bytes() / 1e9:.2f} GB")
+    ch.initial(v, oy, au0, w0)
+    ch.step(warmup)
+    ch.set_timing(True)
+    ch.sync()
+    t0 = time.perf_counter()
+    ch.step(steps)
+    ch.sync()
+    wall = time.perf_counter() - t0
+    kt = ch.kernel_times()
+    ch.set_timing(False)
+    rest = total - warmup - steps
+    if rest > 0:
+        ch.step(rest)
+    st = ch.get_state()
+    fl = ch.fields()
+    ch.close()
+    per_kernel = {k: dict(total_ms=round(ms, 3), launches=n, avg_ms=round(ms / n, 4) if n else None)
+                  for k, (ms, n) in kt.items() if n}
+    return dict(ms_per_step=wall * 1e3 / steps, per_kernel=per_kernel, state=st, fields=fl,
+                setup_s=t_setup)
+
+
+def field_errors(cfg, mixed, ref):
+    """Errors of the mixed form against the FP64 form after the same steps."""
+    from channel_flow_amd import ic as icmod
+    def l2(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out = {}
+    out["v"] = l2(mixed["state"]["vnp1"], ref["state"]["vnp1"])
+    out["omega_y"] = l2(mixed["state"]["omegaynp1"], ref["state"]["omegaynp1"])
+    for k, n in (("unp1", "u"), ("wnp1", "w"), ("omegaxnp1", "omega_x"), ("omegaznp1", "omega_z")):
+        out[n] = l2(mixed["fields"][k], ref["fields"][k])
+    N = cfg["nnz"]
+    kk = np.arange(N + 1)
+    # wall shear dU/dy / re at y = +1 and -1 (T_k'(+-1) = (+-1)^(k+1) k^2)
+    def shear(a):
+        return (np.sum(a * kk ** 2).real / cfg["re"], np.sum(a * kk ** 2 * (-1.0) ** (kk + 1)).real / cfg["re"])
+    sm, sr = shear(mixed["state"]["au0"]), shear(ref["state"]["au0"])
+    out["wall_shear_top"] = abs(sm[0] - sr[0]) / abs(sr[0])
+    out["wall_shear_bottom"] = abs(sm[1] - sr[1]) / abs(sr[1])
+    y = np.cos(np.pi * kk / N)
+    Um = np.polynomial.chebyshev.chebval(y, mixed["state"]["au0"].real)
+    Ur = np.polynomial.chebyshev.chebval(y, ref["state"]["au0"].real)
+    out["mean_profile"] = float(np.max(np.abs(Um - Ur)) / np.max(np.abs(Ur)))
+    # divergence i alpha u + Dv + i beta w of the mixed fields, relative to |Dv|
+    alpha, beta = icmod.wavenumbers(cfg["nnx"], cfg["nny"], cfg["aphi"], cfg["beta"])
+    dv = icmod.cheb_deriv(mixed["state"]["vnp1"])
+    div = 1j * alpha[None, None, :] * mixed["fields"]["unp1"] + dv + \
+        1j * beta[None, :, None] * mixed["fields"]["wnp1"]
+    div[:, cfg["nny"] // 2, :] = 0
+    div[:, 0, 0] = 0
+    out["max_divergence_rel"] = float(np.max(np.abs(div)) / max(np.max(np.abs(dv)), 1e-300))
+    return {k: float("%.3e" % v) for k, v in out.items()}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg3_256x193x256")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--error-steps", type=int, default=100,
+                    help="steps after which the mixed form is compared with the FP64 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
@@ -147,72 +223,60 @@ def main(argv=None):
         return 2
 
     from channel_flow_amd import ic
-    from channel_flow_amd.channel import Channel
 
     cfg = dict(ic.CONFIGS[a.config])
     d = dims(cfg)
     log = (lambda *x: print(*x, file=sys.stderr, flush=True)) if a.verbose else (lambda *x: None)
     t0 = time.perf_counter()
-    v, oy, au0, w0 = ic.seeded_ic(cfg["nnx"], cfg["nny"], cfg["nnz"], cfg["aphi"], cfg["beta"],
-                                  cfg["u_c"], cfg["rms"], cfg["kmax"], cfg["slope"], cfg["seed"])
+    ic_arrays = ic.seeded_ic(cfg["nnx"], cfg["nny"], cfg["nnz"], cfg["aphi"], cfg["beta"], cfg["u_c"],
+                             cfg["rms"], cfg["kmax"], cfg["slope"], cfg["seed"])
     log(f"ic {time.perf_counter() - t0:.2f}s")
-    t0 = time.perf_counter()
-    ch = Channel(cfg["nnx"], cfg["nny"], cfg["nnz"], cfg["aphi"], cfg["beta"], cfg["dt"], cfg["re"],
-                 cfg["re_p"], precision=a.precision)
-    t_setup = time.perf_counter() - t0
-    log(f"setup {t_setup:.2f}s, device bytes {ch.device_bytes() / 1e9:.2f} GB")
-    ch.initial(v, oy, au0, w0)
-    ch.step(a.warmup)
-    ch.sync()
-    log("warmup done")
-    ch.set_timing(True)
-    ch.sync()
-    t0 = time.perf_counter()
-    ch.step(a.steps)
-    ch.sync()
-    wall = time.perf_counter() - t0
-    kt = ch.kernel_times()
-    ch.set_timing(False)
-    st = ch.get_state()
-    finite = all(np.all(np.isfinite(st[k])) for k in ("vnp1", "omegaynp1", "au0"))
-    ms_per_step = wall * 1e3 / a.steps
-    work = kernel_work(d, 16)
-    per_kernel = {k: dict(total_ms=round(ms, 3), launches=n, avg_ms=round(ms / n, 4) if n else None)
-                  for k, (ms, n) in kt.items() if n}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["total_ms"])
-    roof = roofline_of(dom, per_kernel[dom]["avg_ms"], work)
+    forms = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    total = max(a.error_steps, a.warmup + a.steps) if a.precision == "both" else a.warmup + a.steps
+    runs = {f: run_form(cfg, ic_arrays, f, a.steps, a.warmup, total, log) for f in forms}
+    head = "mixed" if "mixed" in runs else "fp64"
+    r = runs[head]
+    finite = all(np.all(np.isfinite(r["state"][k])) for k in ("vnp1", "omegaynp1", "au0"))
+    work = kernel_work(d, head)
+    pk = r["per_kernel"]
+    dom = max(pk, key=lambda k: pk[k]["total_ms"])
+    roof = roofline_of(dom, pk[dom]["avg_ms"], work, head)
     roof["traffic"] = None
-    kernel_roofs = {k: roofline_of(k, per_kernel[k]["avg_ms"], work) for k in per_kernel}
-    log(json.dumps(kernel_roofs, indent=1))
-    cpu = None
-    if not a.no_cpu_baseline:
-        cpu = cpu_baseline(cfg)
+    for f, rr in runs.items():
+        w = kernel_work(d, f)
+        log(f, json.dumps({k: roofline_of(k, rr["per_kernel"][k]["avg_ms"], w, f)
+                           for k in rr["per_kernel"]}, indent=1))
+    cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
     out = {
         "metric": "DNS timesteps/s at 256x193x256 (mixed vs FP64 GPU form, %HBM/MFMA roofline)",
-        "value": round(1e3 / ms_per_step, 3),
+        "value": round(1e3 / r["ms_per_step"], 3),
         "unit": "timesteps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32/bf16-split transforms, f64 solves",
+        "dtype": ("f32 transforms and products, f64 solves and state" if head == "mixed" else "f64"),
         "data": "synthetic: seeded divergence-free Fourier-Chebyshev perturbation on a parabola",
         "config": {"workload": f"{a.config}: nnx={cfg['nnx']} nny={cfg['nny']} N={cfg['nnz']} "
                                f"aphi={cfg['aphi']} beta={cfg['beta']} re={cfg['re']} "
                                f"re_p={cfg['re_p']} dt={cfg['dt']}",
-                   "precision": a.precision, "padded_grid": f"{d['Lx']}x{d['Py']}x{d['Lz']}",
+                   "form": head, "padded_grid": f"{d['Lx']}x{d['Py']}x{d['Lz']}",
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels": per_kernel,
-        "setup_s": round(t_setup, 2),
+        "kernels": pk,
+        "setup_s": round(r["setup_s"], 2),
         "state_finite": bool(finite),
     }
+    if "fp64" in runs and "mixed" in runs:
+        out["fp64_value"] = round(1e3 / runs["fp64"]["ms_per_step"], 3)
+        out["mixed_over_fp64"] = round(runs["fp64"]["ms_per_step"] / runs["mixed"]["ms_per_step"], 3)
+        out["fp64_kernels"] = runs["fp64"]["per_kernel"]
+        out["errors_vs_fp64"] = dict(field_errors(cfg, runs["mixed"], runs["fp64"]), steps=total)
     print(json.dumps(out), flush=True)
-    ch.close()
     return 0
 
 

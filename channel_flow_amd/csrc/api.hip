@@ -13,7 +13,7 @@ using namespace chf;
 namespace chf {
 const char* const kKernelNames[K_COUNT] = {
     "synth_padded", "zfft_inv", "xpass", "zfft_fwd", "analysis", "hassemble",
-    "synth_points", "mean_points", "rhs_v", "rhs_o", "solve"};
+    "synth_points", "mean_points", "rhs_v", "rhs_o", "solve", "rhs_prep"};
 }
 
 namespace {
@@ -149,8 +149,7 @@ void free_all(Ctx* c) {
   f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
   f(c->invV); f(c->invW); f(c->twx); f(c->twz); f(c->costab);
-  for (ChebOp* op : {&c->op_rv, &c->op_ro})
-    for (double*& q : op->frag) { f(q); q = nullptr; }
+  f(c->gv); f(c->go);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -220,6 +219,8 @@ int build(Ctx* c) {
   CHF_TRY(dalloc(c, &c->rvO, (size_t)d.ne * d.ncol));
   CHF_TRY(dalloc(c, &c->roE, (size_t)d.ne * d.ncol));
   CHF_TRY(dalloc(c, &c->roO, (size_t)d.ne * d.ncol));
+  CHF_TRY(dalloc(c, &c->gv, fld));
+  CHF_TRY(dalloc(c, &c->go, fld));
   CHF_TRY(dalloc(c, &c->invV, (size_t)d.ngroups * d.inv_stride));
   CHF_TRY(dalloc(c, &c->invW, (size_t)d.ngroups * d.inv_stride));
 
@@ -274,18 +275,7 @@ int build(Ctx* c) {
     CHF_TRY(dalloc(c, &c->costabf, cf.size()));
     CHF_TRY(hipMemcpy(c->costabf, cf.data(), cf.size() * sizeof(float), hipMemcpyHostToDevice));
   }
-  CHF_TRY(pack_op(c, c->op_rv, ne - 2, ne, 6, [&](int m, int r, int j) {
-    const int par = m / 3, which = m % 3, l = 2 * j + par;
-    if (r >= (par ? no - 2 : ne - 2) || l > N) return 0.0;
-    const std::vector<double>& T = which == 0 ? t4 : (which == 1 ? t2 : t0);
-    return T[(size_t)(2 + r) * Ny + l];
-  }));
-  CHF_TRY(pack_op(c, c->op_ro, ne - 1, ne, 4, [&](int m, int r, int j) {
-    const int par = m / 2, which = m % 2, l = 2 * j + par;
-    if (r >= (par ? no - 1 : ne - 1) || l > N) return 0.0;
-    const std::vector<double>& T = which == 0 ? t2 : t0;
-    return T[(size_t)(1 + r) * Ny + l];
-  }));
+
 
   // FFT twiddles e^{-2 pi i m/L}
   auto twiddles = [](int L) {
@@ -378,6 +368,7 @@ int nonlinear(Ctx* c) {
 }
 
 int solve(Ctx* c) {
+  CHF_TRY(timed(c, K_RHS_PREP, [&] { return launch_rhs_prep(*c); }));
   CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
   CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
   CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c, 0); }));

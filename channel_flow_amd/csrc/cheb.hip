@@ -192,111 +192,142 @@ __device__ __forceinline__ double col_k2(long col, int Mx, const double* al, con
   return al[kx] * al[kx] + be[kz] * be[kz];
 }
 
-// v equation rows at points p = 2 + r: c T4 + (1 - k^2 dt/re) T2 - k^2 (1 - k^2 c) T0
-struct RhsVP {
-  static constexpr int NACC = 6, NB = 2;  // (E: T4, T2, T0), (O: T4, T2, T0)
-  static constexpr int bsel(int m) { return m / 3; }
-  const double* frag[6];
-  MfmaShape sh;  // M = ne - 2, K = ne
-  const double* vin;
+// The explicit Crank-Nicolson operators are polynomials in D^2 evaluated at
+// the collocation points: T2 = T0 D^2 and T4 = T0 D^4 as operators on
+// coefficient vectors, so
+//   B_v v = T0 [c D^4 v + (1 - 2 c k^2) D^2 v - k^2 (1 - c k^2) v],
+//   B_w w = T0 [c D^2 w + (1 - c k^2) w],           c = dt / (2 re).
+// k_rhs_prep forms the bracketed coefficient vectors with one backward
+// derivative sweep per column; the right-hand sides are then cos-table
+// syntheses at the interior points plus the AB2 forcing
+// 3 dt/2 h^n - dt/2 h^{n-1} (PF:252-257, 335-340), folded by parity.
+struct PrepArgs {
+  const double2 *v, *oy;
+  double2 *gv, *go;
+  const double *al, *be;
+  int Mx, N;
+  long ncol;
+  double c;
+};
+
+__global__ __launch_bounds__(256) void k_rhs_prep(PrepArgs a) {
+  const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cm >= a.ncol) return;
+  const int kz = (int)(cm / a.Mx);
+  const int kx = (int)(cm - (long)kz * a.Mx);
+  const double s = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
+  const double c = a.c, c2 = 1.0 - 2.0 * c * s, c0 = s * (1.0 - c * s), cw = 1.0 - c * s;
+  const int N = a.N;
+  // derivative levels 1..4 of v and 1..2 of omega_y at k+1 and k+2 (unhalved)
+  double2 v1[2] = {}, v2[2] = {}, v3[2] = {}, v4[2] = {}, w1[2] = {}, w2[2] = {};
+  double2 vn = make_double2(0, 0), wn = make_double2(0, 0);  // a_{k+1}
+  auto step = [](double2 up2, double2 lowp1, int k) {          // b_k = b_{k+2} + 2(k+1) a_{k+1}
+    const double f = 2.0 * (k + 1);
+    return make_double2(up2.x + f * lowp1.x, up2.y + f * lowp1.y);
+  };
+  for (int k = N; k >= 0; --k) {
+    const long off = (long)k * a.ncol + cm;
+    const double2 vk = a.v[off], wk = a.oy[off];
+    double2 d1, d2, d3, d4, e1, e2;
+    if (k == N) {
+      d1 = d2 = d3 = d4 = e1 = e2 = make_double2(0, 0);
+    } else {
+      d1 = step(v1[1], vn, k);
+      d2 = step(v2[1], v1[0], k);
+      d3 = step(v3[1], v2[0], k);
+      d4 = step(v4[1], v3[0], k);
+      e1 = step(w1[1], wn, k);
+      e2 = step(w2[1], w1[0], k);
+    }
+    v1[1] = v1[0]; v1[0] = d1;
+    v2[1] = v2[0]; v2[0] = d2;
+    v3[1] = v3[0]; v3[0] = d3;
+    v4[1] = v4[0]; v4[0] = d4;
+    w1[1] = w1[0]; w1[0] = e1;
+    w2[1] = w2[0]; w2[0] = e2;
+    if (k == 0) {  // c_0 = 2
+      d2.x *= 0.5; d2.y *= 0.5; d4.x *= 0.5; d4.y *= 0.5; e2.x *= 0.5; e2.y *= 0.5;
+    }
+    a.gv[off] = make_double2(c * d4.x + c2 * d2.x - c0 * vk.x, c * d4.y + c2 * d2.y - c0 * vk.y);
+    a.go[off] = make_double2(c * e2.x + cw * wk.x, c * e2.y + cw * wk.y);
+    vn = vk;
+    wn = wk;
+  }
+}
+
+// rows r = 0.. at points p = r + POFF (POFF = 2 for v, 1 for omega_y)
+template <int POFF>
+struct RhsSynP {
+  using TC = double;
+  static constexpr int NACC = 2, NB = 2;
+  static constexpr int bsel(int m) { return m; }
+  const double* costab;
+  int n4;
+  MfmaShape sh;  // M = ne - POFF, K = ne
+  const double* g;
   const double *hn, *hp;
   double *outE, *outO;
-  const double *al, *be;
-  int N, ne, no, Mx;
+  int N, no;
   long ncold;
-  double dt, re;
+  double dt;
   __device__ void select(int) {}
+  __device__ int aindex(int m, int row, int k0, int n4_) const {
+    return (int)((2L * (row + POFF) * (2 * k0 + m)) % n4_);   // T_l(y_p) = cos(pi p l / N)
+  }
+  __device__ int astep(int, int row, int n4_) const { return (int)((16L * (row + POFF)) % n4_); }
   __device__ void loadB(int j, long col, double* v) const {
-    v[0] = vin[(2 * j) * ncold + col];
-    v[1] = (2 * j + 1 <= N) ? vin[(2 * j + 1) * ncold + col] : 0.0;
+    v[0] = g[(2 * j) * ncold + col];
+    v[1] = (2 * j + 1 <= N) ? g[(2 * j + 1) * ncold + col] : 0.0;
   }
   __device__ void store(int r, long col, const double* acc) const {
-    const double k2 = col_k2(col, Mx, al, be);
-    const double c = dt * 0.5 / re;
-    const double c2 = 1.0 - k2 * dt / re;
-    const double c0 = k2 * (1.0 - k2 * c);
-    const int p = 2 + r, q = N - p;
+    const int p = r + POFF, q = N - p;
     const double fp = 3.0 * dt / 2.0 * hn[p * ncold + col] - dt / 2.0 * hp[p * ncold + col];
     const double fq = 3.0 * dt / 2.0 * hn[q * ncold + col] - dt / 2.0 * hp[q * ncold + col];
     const double sym = (p == q) ? fp : 0.5 * (fp + fq);
-    outE[r * ncold + col] = c * acc[0] + c2 * acc[1] - c0 * acc[2] + sym;
-    if (r < no - 2) outO[r * ncold + col] = c * acc[3] + c2 * acc[4] - c0 * acc[5] + 0.5 * (fp - fq);
+    outE[r * ncold + col] = acc[0] + sym;
+    if (r < no - POFF) outO[r * ncold + col] = acc[1] + 0.5 * (fp - fq);
   }
 };
 
-// omega_y rows at points p = 1 + r: c T2 + (1 - k^2 c) T0
-struct RhsOP {
-  static constexpr int NACC = 4, NB = 2;  // (E: T2, T0), (O: T2, T0)
-  static constexpr int bsel(int m) { return m / 2; }
-  const double* frag[4];
-  MfmaShape sh;  // M = ne - 1, K = ne
-  const double* oin;
-  const double *hn, *hp;
-  double *outE, *outO;
-  const double *al, *be;
-  int N, ne, no, Mx;
-  long ncold;
-  double dt, re;
-  __device__ void select(int) {}
-  __device__ void loadB(int j, long col, double* v) const {
-    v[0] = oin[(2 * j) * ncold + col];
-    v[1] = (2 * j + 1 <= N) ? oin[(2 * j + 1) * ncold + col] : 0.0;
-  }
-  __device__ void store(int r, long col, const double* acc) const {
-    const double k2 = col_k2(col, Mx, al, be);
-    const double c = dt * 0.5 / re;
-    const double c0 = 1.0 - k2 * c;
-    const int p = 1 + r, q = N - p;
-    const double fp = 3.0 * dt / 2.0 * hn[p * ncold + col] - dt / 2.0 * hp[p * ncold + col];
-    const double fq = 3.0 * dt / 2.0 * hn[q * ncold + col] - dt / 2.0 * hp[q * ncold + col];
-    const double sym = (p == q) ? fp : 0.5 * (fp + fq);
-    outE[r * ncold + col] = c * acc[0] + c0 * acc[1] + sym;
-    if (r < no - 1) outO[r * ncold + col] = c * acc[2] + c0 * acc[3] + 0.5 * (fp - fq);
-  }
-};
+hipError_t launch_rhs_prep(Ctx& c) {
+  PrepArgs a{};
+  a.v = c.v;
+  a.oy = c.oy;
+  a.gv = c.gv;
+  a.go = c.go;
+  a.al = c.alpha;
+  a.be = c.beta;
+  a.Mx = c.d.Mx;
+  a.N = c.d.N;
+  a.ncol = c.d.ncol;
+  a.c = c.prm.dt * 0.5 / c.prm.re;
+  hipLaunchKernelGGL(k_rhs_prep, dim3((unsigned)((a.ncol + 255) / 256)), dim3(256), 0, c.stream, a);
+  return hipGetLastError();
+}
+
+template <int POFF>
+static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const double2* hp, double2* oE,
+                          double2* oO) {
+  RhsSynP<POFF> pb{};
+  pb.costab = c.costab;
+  pb.n4 = 4 * c.d.N;
+  pb.sh = mfma_shape(c.d.ne - POFF, c.d.ne);
+  pb.g = reinterpret_cast<const double*>(g);
+  pb.hn = reinterpret_cast<const double*>(hn);
+  pb.hp = reinterpret_cast<const double*>(hp);
+  pb.outE = reinterpret_cast<double*>(oE);
+  pb.outO = reinterpret_cast<double*>(oO);
+  pb.N = c.d.N;
+  pb.no = c.d.no;
+  pb.ncold = 2 * c.d.ncol;
+  pb.dt = c.prm.dt;
+  return run_mfma_cos(c.stream, pb, 1);
+}
 
 hipError_t launch_rhs(Ctx& c, int which) {
-  const Dims& d = c.d;
-  const int nxt = c.hcur;        // h^n
-  const int prv = 1 - c.hcur;    // h^{n-1}
-  if (which == 0) {
-    RhsVP pv{};
-    for (int m = 0; m < 6; ++m) pv.frag[m] = c.op_rv.frag[m];
-    pv.sh = c.op_rv.sh;
-    pv.vin = reinterpret_cast<const double*>(c.v);
-    pv.hn = reinterpret_cast<const double*>(c.hv[nxt]);
-    pv.hp = reinterpret_cast<const double*>(c.hv[prv]);
-    pv.outE = reinterpret_cast<double*>(c.rvE);
-    pv.outO = reinterpret_cast<double*>(c.rvO);
-    pv.al = c.alpha;
-    pv.be = c.beta;
-    pv.N = d.N;
-    pv.ne = d.ne;
-    pv.no = d.no;
-    pv.Mx = d.Mx;
-    pv.ncold = 2 * d.ncol;
-    pv.dt = c.prm.dt;
-    pv.re = c.prm.re;
-    return run_mfma(c.stream, pv, 1);
-  }
-  RhsOP po{};
-  for (int m = 0; m < 4; ++m) po.frag[m] = c.op_ro.frag[m];
-  po.sh = c.op_ro.sh;
-  po.oin = reinterpret_cast<const double*>(c.oy);
-  po.hn = reinterpret_cast<const double*>(c.hg[nxt]);
-  po.hp = reinterpret_cast<const double*>(c.hg[prv]);
-  po.outE = reinterpret_cast<double*>(c.roE);
-  po.outO = reinterpret_cast<double*>(c.roO);
-  po.al = c.alpha;
-  po.be = c.beta;
-  po.N = d.N;
-  po.ne = d.ne;
-  po.no = d.no;
-  po.Mx = d.Mx;
-  po.ncold = 2 * d.ncol;
-  po.dt = c.prm.dt;
-  po.re = c.prm.re;
-  return run_mfma(c.stream, po, 1);
+  const int nxt = c.hcur, prv = 1 - c.hcur;  // h^n, h^{n-1}
+  if (which == 0) return rhs_syn<2>(c, c.gv, c.hv[nxt], c.hv[prv], c.rvE, c.rvO);
+  return rhs_syn<1>(c, c.go, c.hg[nxt], c.hg[prv], c.roE, c.roO);
 }
 
 // ---------------------------------------------------------------- h assembly
@@ -386,15 +417,19 @@ struct MArgs {
   long ncol;
 };
 
-__global__ void k_mean_points(MArgs a) {
-  const int j = threadIdx.x;
-  for (int jj = j; jj < a.Pn; jj += blockDim.x) {
+__global__ __launch_bounds__(256) void k_mean_points(MArgs a) {
+  __shared__ double2 s1[260], s3[260];
+  for (int l = threadIdx.x; l <= a.N; l += blockDim.x) {
+    s1[l] = a.lx[(long)l * a.ncol];
+    s3[l] = a.lz[(long)l * a.ncol];
+  }
+  __syncthreads();
+  for (int jj = threadIdx.x; jj < a.Pn; jj += blockDim.x) {
     double2 e1 = make_double2(0, 0), o1 = make_double2(0, 0);
     double2 e3 = make_double2(0, 0), o3 = make_double2(0, 0);
     for (int l = 0; l <= a.N; ++l) {
       const double t = a.tpts[jj * a.Ny + l];
-      const double2 x1 = a.lx[(long)l * a.ncol];
-      const double2 x3 = a.lz[(long)l * a.ncol];
+      const double2 x1 = s1[l], x3 = s3[l];
       if (l & 1) {
         o1.x += t * x1.x; o1.y += t * x1.y;
         o3.x += t * x3.x; o3.y += t * x3.y;

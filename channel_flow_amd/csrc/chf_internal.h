@@ -86,11 +86,11 @@ struct Ctx {
   double2 *hvc = nullptr, *hgc = nullptr;  // h coefficients
   // solve work
   double2 *rvE = nullptr, *rvO = nullptr, *roE = nullptr, *roO = nullptr;
+  double2 *gv = nullptr, *go = nullptr;  // explicit-operator coefficient vectors
   // operators
   double* host_mats = nullptr;
   double *tsyn = nullptr, *tana = nullptr, *tpts = nullptr, *t0 = nullptr, *t2 = nullptr,
          *t4 = nullptr;
-  ChebOp op_rv, op_ro;     // packed explicit-operator fragments (rhs)
   double* costab = nullptr;  // cos(pi n / 2N), n < 4N
   float* costabf = nullptr;  // the same rounded to FP32 (mixed form)
   float2 *twxf = nullptr, *twzf = nullptr;  // FP32 twiddles (mixed form)
@@ -125,7 +125,7 @@ bool fft_length_ok(int L);
 // Kernel classes for per-launch timing.
 enum KernelId {
   K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
-  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_COUNT
+  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -143,6 +143,7 @@ hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out
 hipError_t launch_hassemble(Ctx& c);
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_mean_points(Ctx& c);
+hipError_t launch_rhs_prep(Ctx& c);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
 hipError_t launch_solve(Ctx& c, int mode);  // mode 0 = solve, 1 = kinematics only
 hipError_t launch_setup_inverses(Ctx& c);

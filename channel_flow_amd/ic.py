@@ -110,17 +110,26 @@ def seeded_ic(nnx, nny, nnz, aphi, beta, u_c, rms, kmax, slope, seed):
 
 
 # Workload table: the five configurations of the measurement plan.
+# All are in centreline units (U_c = 1, re = centreline Reynolds number), the
+# convention the 32x33x32 case states explicitly (Re_centerline = 3000).  The
+# Re_tau classes map to the centreline Reynolds number of the turbulent flow at
+# that Re_tau (U_c+ = 18.3, 20.1, 21.2 for Re_tau = 180, 395, 590).  re_p = 2/re
+# makes the laminar profile U_c (1 - y^2) an exact steady state of the mean
+# equation.  The noise is 10% of centreline (rms 0.1).  With these units the
+# explicit AB2 advection stays well inside its stability range:
+# kx_max U_c dt = 0.035 at 256x193x256 (wall units, U_c = 20, would give 0.64,
+# where AB2 amplifies the highest streamwise modes by ~30% per step).
 # (nnx, nny, nnz, aphi, beta, re, re_p, dt, U_c, rms, kmax, slope, seed)
 CONFIGS = {
     "cfg1_32x33x32": dict(nnx=32, nny=32, nnz=32, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0,
                           dt=1.0e-3, u_c=1.0, rms=1.0e-3, kmax=4.0, slope=0.0, seed=1001, steps=100),
-    "cfg2_128x129x128": dict(nnx=128, nny=128, nnz=128, aphi=2.0, beta=2.0 / 3.0, re=180.0, re_p=1.0,
-                             dt=5.0e-4, u_c=18.0, rms=1.8, kmax=16.0, slope=-4.0 / 3.0, seed=1002,
-                             steps=1000),
-    "cfg3_256x193x256": dict(nnx=256, nny=256, nnz=192, aphi=1.0, beta=0.5, re=395.0, re_p=1.0,
-                             dt=2.5e-4, u_c=20.0, rms=2.0, kmax=16.0, slope=-4.0 / 3.0, seed=1003,
-                             steps=100),
-    "cfg5_512x257x512": dict(nnx=512, nny=512, nnz=256, aphi=1.0, beta=0.5, re=590.0, re_p=1.0,
-                             dt=2.5e-4, u_c=21.0, rms=2.1, kmax=16.0, slope=-4.0 / 3.0, seed=1005,
-                             steps=50),
+    "cfg2_128x129x128": dict(nnx=128, nny=128, nnz=128, aphi=2.0, beta=2.0 / 3.0, re=3300.0,
+                             re_p=2.0 / 3300.0, dt=5.0e-4, u_c=1.0, rms=0.1, kmax=16.0,
+                             slope=-4.0 / 3.0, seed=1002, steps=1000),
+    "cfg3_256x193x256": dict(nnx=256, nny=256, nnz=192, aphi=1.0, beta=0.5, re=7940.0,
+                             re_p=2.0 / 7940.0, dt=2.5e-4, u_c=1.0, rms=0.1, kmax=16.0,
+                             slope=-4.0 / 3.0, seed=1003, steps=100),
+    "cfg5_512x257x512": dict(nnx=512, nny=512, nnz=256, aphi=1.0, beta=0.5, re=12500.0,
+                             re_p=2.0 / 12500.0, dt=2.5e-4, u_c=1.0, rms=0.1, kmax=16.0,
+                             slope=-4.0 / 3.0, seed=1005, steps=50),
 }

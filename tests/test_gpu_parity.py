@@ -183,3 +183,46 @@ def test_mixed_form_against_fp64_and_oracle(require_gpu):
         assert e1[k] < 1e-6, (k, e1[k])
     for k in ("au0", "w0"):
         assert e1[k] < 1e-10, (k, e1[k])
+
+
+@pytest.mark.parametrize("nnz", [96, 192])
+def test_large_degree_small_box(require_gpu, nnz):
+    """N = 96 and 192 (the multi-row-block contraction paths used at the
+    256x193x256 workload) on a 16x16 box: initial() + 3 steps vs the oracle."""
+    cfg = dict(nnx=16, nny=16, nnz=nnz, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
+               dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=21)
+    p = params_of(cfg)
+    v, oy, au0, w0 = ic_of(cfg)
+    st = O.initial_presolve_state(v, oy, au0, w0, p)
+    with make_channel(cfg) as ch:
+        ch.initial(v, oy, au0, w0)
+        e0 = state_errors(ch.get_state(), st)
+        report(f"N={nnz} initial", e0)
+        O.Stepper(p).step(st, 3)
+        ch.step(3)
+        e1 = state_errors(ch.get_state(), st)
+    report(f"N={nnz} 3 steps", e1)
+    for k, e in e0.items():
+        assert e < ONE_OP_TOL, (k, e)
+    for k, e in e1.items():
+        assert e < TRAJ_TOL, (k, e)
+
+
+def test_mixed_large_degree(require_gpu):
+    """Mixed form at N = 192 on a 32x32 box (the wide contraction tiles and
+    FP32 paths of the 256x193x256 workload): 5 steps against the FP64 form."""
+    cfg = dict(nnx=32, nny=32, nnz=192, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
+               dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=22)
+    v, oy, au0, w0 = ic_of(cfg)
+    with make_channel(cfg, "mixed") as mx, make_channel(cfg, "fp64") as fp:
+        mx.initial(v, oy, au0, w0)
+        fp.initial(v, oy, au0, w0)
+        mx.step(5)
+        fp.step(5)
+        gm, gf = mx.get_state(), fp.get_state()
+    e = state_errors(gm, gf)
+    report("mixed N=192 vs fp64, 5 steps", e)
+    for k in ("vnp1", "omegaynp1"):
+        assert e[k] < 1e-5, (k, e[k])
+    for k in ("au0", "w0"):
+        assert e[k] < 1e-9, (k, e[k])
