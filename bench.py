@@ -1,14 +1,15 @@
 """Timestep throughput of the MI355X channel-flow core (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--config cfg3_256x193x256]
+    python bench.py [--steps K] [--warmup W] [--config retau395_256x193x256]
                     [--precision fp64] [--no-cpu-baseline]
 
 A step is one pass of the per-timestep core (PF:220-532): per-mode v/omega_y
 solves, kinematics and mean mode, then getnonlinear (six 3/2-dealiased
 products, Lamb vector, h assembly, AB2 rotation).  Inputs are resident in HBM
 when the timed region starts.  The default workload is the 256 x 193 x 256
-channel (nnx = nny = 256, N = 192, Lx = 2 pi, Lz = pi, Re_tau ~ 395 class) from
-a seeded divergence-free initial condition.  Prints one JSON line.
+channel (nnx = nny = 256, N = 192, Lx = 2 pi, Lz = pi, Re_tau ~ 395 class, in
+centreline units: U_c = 1, re = 7940, dt = 2.5e-4) from a seeded
+divergence-free initial condition.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -89,6 +90,46 @@ def roofline_of(name, avg_ms, work, precision="fp64"):
     return dict(kernel=name, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), avg_ms=round(avg_ms, 4),
                 algorithmic_bytes=int(nbytes))
+
+
+# HIP kernel symbol behind each timed kernel class (the FP32 forms share symbols
+# with the FP64 ones; template arguments differ).
+KERNEL_SYMBOL = {"synth_padded": "k_mfma_cos", "zfft_inv": "k_zfft", "xpass": "k_xpass",
+                 "zfft_fwd": "k_zfft", "analysis": "k_mfma_cos", "hassemble": "k_hassemble",
+                 "synth_points": "k_mfma_cos", "mean_points": "k_mean_points",
+                 "rhs_prep": "k_rhs_prep", "rhs_v": "k_mfma_cos", "rhs_o": "k_mfma_cos",
+                 "solve": "k_solve"}
+
+
+def measured_traffic(config, form, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a device copy of known
+    size), or None when no summary for this workload and form is present or
+    the kernel's symbol is shared by several kernel classes."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{config}_{form}.json")
+    sym = KERNEL_SYMBOL.get(kernel)
+    if not os.path.exists(path) or sym is None or list(KERNEL_SYMBOL.values()).count(sym) != 1:
+        return None
+    with open(path) as f:
+        data = json.load(f)["kernels"]
+    hits = [v for k, v in data.items() if k.split("<")[0].split()[-1] == sym]
+    if len(hits) != 1 or hits[0]["fetch_bytes"] is None or hits[0]["write_bytes"] is None:
+        return None
+    return int(hits[0]["fetch_bytes"] + hits[0]["write_bytes"])
+
+
+def copy_probe(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of `nbytes`, run under the FETCH_SIZE and
+    WRITE_SIZE counters to calibrate them (the copy moves exactly nbytes each
+    way)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+    return 0
 
 
 def cpu_baseline(cfg, seconds_budget=30.0):
@@ -210,13 +251,17 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg3_256x193x256")
+    ap.add_argument("--config", default="retau395_256x193x256")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--error-steps", type=int, default=100,
                     help="steps after which the mixed form is compared with the FP64 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a 1 GiB device copy (counter calibration)")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
+    if a.copy_probe:
+        return copy_probe()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the whole channel lives on one device "
               "(multi-GPU sharding is a design note, not a run)", file=sys.stderr)
@@ -241,14 +286,14 @@ def main(argv=None):
     pk = r["per_kernel"]
     dom = max(pk, key=lambda k: pk[k]["total_ms"])
     roof = roofline_of(dom, pk[dom]["avg_ms"], work, head)
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(a.config, head, dom)
     for f, rr in runs.items():
         w = kernel_work(d, f)
         log(f, json.dumps({k: roofline_of(k, rr["per_kernel"][k]["avg_ms"], w, f)
                            for k in rr["per_kernel"]}, indent=1))
     cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
     out = {
-        "metric": "DNS timesteps/s at 256x193x256 (mixed vs FP64 GPU form, %HBM/MFMA roofline)",
+        "metric": "DNS timesteps/s at 256\u00d7193\u00d7256, mixed vs FP64 GPU form, %HBM/MFMA roofline",
         "value": round(1e3 / r["ms_per_step"], 3),
         "unit": "timesteps/s",
         "n_gpus": 1,

@@ -121,15 +121,15 @@ def seeded_ic(nnx, nny, nnz, aphi, beta, u_c, rms, kmax, slope, seed):
 # where AB2 amplifies the highest streamwise modes by ~30% per step).
 # (nnx, nny, nnz, aphi, beta, re, re_p, dt, U_c, rms, kmax, slope, seed)
 CONFIGS = {
-    "cfg1_32x33x32": dict(nnx=32, nny=32, nnz=32, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0,
+    "re3000_32x33x32": dict(nnx=32, nny=32, nnz=32, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0,
                           dt=1.0e-3, u_c=1.0, rms=1.0e-3, kmax=4.0, slope=0.0, seed=1001, steps=100),
-    "cfg2_128x129x128": dict(nnx=128, nny=128, nnz=128, aphi=2.0, beta=2.0 / 3.0, re=3300.0,
+    "retau180_128x129x128": dict(nnx=128, nny=128, nnz=128, aphi=2.0, beta=2.0 / 3.0, re=3300.0,
                              re_p=2.0 / 3300.0, dt=5.0e-4, u_c=1.0, rms=0.1, kmax=16.0,
                              slope=-4.0 / 3.0, seed=1002, steps=1000),
-    "cfg3_256x193x256": dict(nnx=256, nny=256, nnz=192, aphi=1.0, beta=0.5, re=7940.0,
+    "retau395_256x193x256": dict(nnx=256, nny=256, nnz=192, aphi=1.0, beta=0.5, re=7940.0,
                              re_p=2.0 / 7940.0, dt=2.5e-4, u_c=1.0, rms=0.1, kmax=16.0,
                              slope=-4.0 / 3.0, seed=1003, steps=100),
-    "cfg5_512x257x512": dict(nnx=512, nny=512, nnz=256, aphi=1.0, beta=0.5, re=12500.0,
+    "retau590_512x257x512": dict(nnx=512, nny=512, nnz=256, aphi=1.0, beta=0.5, re=12500.0,
                              re_p=2.0 / 12500.0, dt=2.5e-4, u_c=1.0, rms=0.1, kmax=16.0,
                              slope=-4.0 / 3.0, seed=1005, steps=50),
 }

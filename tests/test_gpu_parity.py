@@ -101,9 +101,9 @@ def test_nonlinear_matches_oracle(small_case):
 
 
 @pytest.mark.parametrize("nsteps", [10, 100])
-def test_cfg1_trajectory(require_gpu, nsteps):
+def test_32x33x32_trajectory(require_gpu, nsteps):
     """Config 1 (32x33x32, Re_c = 3000, dt = 1e-3): n steps against the oracle."""
-    cfg = ic.CONFIGS["cfg1_32x33x32"]
+    cfg = ic.CONFIGS["re3000_32x33x32"]
     p = params_of(cfg)
     v, oy, au0, w0 = ic_of(cfg)
     st = O.initial_presolve_state(v, oy, au0, w0, p)
@@ -113,7 +113,7 @@ def test_cfg1_trajectory(require_gpu, nsteps):
         ch.step(nsteps)
         got = ch.get_state()
     errs = state_errors(got, st)
-    report(f"cfg1 {nsteps} steps", errs)
+    report(f"32x33x32 {nsteps} steps", errs)
     for k, e in errs.items():
         assert e < TRAJ_TOL, (k, e)
 
@@ -162,7 +162,7 @@ def test_mixed_form_against_fp64_and_oracle(require_gpu):
     Chebyshev degrees by up to ~N^2 (max-norm ~1e-3 at N = 32).  The FP64
     solve damps those degrees, so v and omega_y after 10 steps agree with the
     FP64 form to ~1e-7 and the mean flow to ~1e-13."""
-    cfg = ic.CONFIGS["cfg1_32x33x32"]
+    cfg = ic.CONFIGS["re3000_32x33x32"]
     p = params_of(cfg)
     v, oy, au0, w0 = ic_of(cfg)
     ref = O.initial_presolve_state(v, oy, au0, w0, p)
