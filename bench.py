@@ -71,7 +71,8 @@ def kernel_work(d, precision="fp64"):
                   2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
         "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S,
                   2.0 * ((ne - 1) * ne + (no - 1) * no) * col),
-        "solve": (inv_bytes + (rv_rows + ro_rows) / Ny * S + 6 * S, 0.0),
+        "solve": (inv_bytes + (rv_rows + ro_rows) / Ny * S + 2 * S, 0.0),
+        "kinematics": (6 * S, 0.0),
     }
 
 
@@ -98,7 +99,7 @@ KERNEL_SYMBOL = {"synth_padded": "k_mfma_cos", "zfft_inv": "k_zfft", "xpass": "k
                  "zfft_fwd": "k_zfft", "analysis": "k_mfma_cos", "hassemble": "k_hassemble",
                  "synth_points": "k_mfma_cos", "mean_points": "k_mean_points",
                  "rhs_prep": "k_rhs_prep", "rhs_v": "k_mfma_cos", "rhs_o": "k_mfma_cos",
-                 "solve": "k_solve"}
+                 "solve": "k_solve", "kinematics": "k_kinematics"}
 
 
 def measured_traffic(config, form, kernel):

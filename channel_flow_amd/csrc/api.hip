@@ -13,7 +13,7 @@ using namespace chf;
 namespace chf {
 const char* const kKernelNames[K_COUNT] = {
     "synth_padded", "zfft_inv", "xpass", "zfft_fwd", "analysis", "hassemble",
-    "synth_points", "mean_points", "rhs_v", "rhs_o", "solve", "rhs_prep"};
+    "synth_points", "mean_points", "rhs_v", "rhs_o", "solve", "rhs_prep", "kinematics"};
 }
 
 namespace {
@@ -371,7 +371,8 @@ int solve(Ctx* c) {
   CHF_TRY(timed(c, K_RHS_PREP, [&] { return launch_rhs_prep(*c); }));
   CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
   CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
-  CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c, 0); }));
+  CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c); }));
+  CHF_TRY(timed(c, K_KINEMATICS, [&] { return launch_kinematics(*c); }));
   return 0;
 }
 
@@ -472,7 +473,7 @@ int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1, cons
     CHF_TRY(hipMemsetAsync(q, 0, mb, c->stream));
   // fields with all-zero u, w, omega_x, omega_z at the Nyquist column
   for (double2* q : {c->u, c->w, c->ox, c->oz}) CHF_TRY(hipMemsetAsync(q, 0, fb, c->stream));
-  CHF_TRY(launch_solve(*c, 1));
+  CHF_TRY(launch_kinematics(*c));
   int rc = nonlinear(c);
   if (rc) return rc;
   CHF_TRY(hipStreamSynchronize(c->stream));

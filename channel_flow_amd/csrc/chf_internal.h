@@ -125,7 +125,7 @@ bool fft_length_ok(int L);
 // Kernel classes for per-launch timing.
 enum KernelId {
   K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
-  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_COUNT
+  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_KINEMATICS, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -145,7 +145,8 @@ hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const*
 hipError_t launch_mean_points(Ctx& c);
 hipError_t launch_rhs_prep(Ctx& c);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
-hipError_t launch_solve(Ctx& c, int mode);  // mode 0 = solve, 1 = kinematics only
+hipError_t launch_solve(Ctx& c);       // v, omega_y and the mean mode
+hipError_t launch_kinematics(Ctx& c);  // u, w, omega_x, omega_z from v, omega_y, U, W
 hipError_t launch_setup_inverses(Ctx& c);
 
 }  // namespace chf

@@ -164,273 +164,263 @@ hipError_t launch_setup_inverses(Ctx& c) {
 struct SolveArgs {
   const double2 *rvE, *rvO, *roE, *roO;  // parity-folded right-hand sides
   const double *invV, *invW;
-  double2 *v, *oy, *u, *w, *ox, *oz;
+  double2 *v, *oy;
   double2 *au0, *w0;
   const double2 *c1n, *c1p, *c3n, *c3p;  // mean Lamb point values, n and n-1
   const double *t0, *t2;
-  const double *al, *be;
   int Mx, Mz, N, Ny, ne, no;
   long stride, ncol;
   double dt, re;
-  int mode;  // 0 = solve + kinematics, 1 = kinematics only
 };
 
-// Backward recurrence for the derivative of 4 real coefficient vectors held
-// in LDS (x[q*Ny + k]) into d[q*Ny + k]; lanes 0..nv-1 each own one vector.
-__device__ void lds_deriv(const double* x, double* d, int nv, int N, int Ny) {
-  const int q = threadIdx.x;
-  if (q < nv) {
-    const double* a = x + q * Ny;
-    double* b = d + q * Ny;
-    double b2 = 0.0, b1 = 0.0;
-    b[N] = 0.0;
-    b1 = 0.0;
-    for (int k = N - 1; k >= 0; --k) {
-      const double bk = b2 + 2.0 * (k + 1) * a[k + 1];
-      b2 = b1;
-      b1 = bk;
-      b[k] = (k == 0) ? 0.5 * bk : bk;
+// Block -> (kx, |kz|) group.  Consecutive blocks are dealt round-robin to the
+// eight XCDs, each with its own L2; the remap gives every XCD a contiguous run
+// of groups, so the neighbouring kx columns whose 16-byte values share a cache
+// line are read and written through the same L2.
+__device__ __forceinline__ int xcd_group(int b, int nb) {
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// One block per (kx, |kz|) group; modes m0 = (kz, kx), m1 = (Mz - kz, kx) share
+// the operator.  The block streams the group's four half-system inverses once,
+// multiplying each by the 4 real right-hand-side vectors (Re, Im of both modes),
+// and writes v^{n+1} and omega_y^{n+1} (PF:242-394).  Group 0 is the mean mode
+// (PF:401-474): U and W with the omega_y-type operator at k^2 = 0.
+__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
+  extern __shared__ double rb[];  // [2 eq][4 vec][Ny] right-hand sides
+  const int Ny = a.Ny, N = a.N, ne = a.ne, no = a.no;
+  const int g = xcd_group(blockIdx.x, gridDim.x);
+  const int kx = g % a.Mx, kzab = g / a.Mx;
+  const bool mean = (g == 0);
+  const int kz0 = kzab, kz1 = kzab ? a.Mz - kzab : 0;
+  const long col0 = (long)kz0 * a.Mx + kx, col1 = (long)kz1 * a.Mx + kx;
+  const double c = a.dt * 0.5 / a.re;
+
+  // ---- right-hand sides, parity-folded (rows of the half systems)
+  if (!mean) {
+    for (int idx = threadIdx.x; idx < 2 * Ny; idx += blockDim.x) {
+      const int eq = idx / Ny, row = idx - eq * Ny;
+      // row < ne: even system row; else odd system row (row - ne)
+      const bool odd = row >= ne;
+      const int i = odd ? row - ne : row;
+      const int nbc = eq ? 1 : 2;
+      double2 x0 = make_double2(0, 0), x1 = make_double2(0, 0);
+      if (i >= nbc) {
+        const int r = i - nbc;
+        const double2* src = eq ? (odd ? a.roO : a.roE) : (odd ? a.rvO : a.rvE);
+        x0 = src[(long)r * a.ncol + col0];
+        if (kzab > 0) x1 = src[(long)r * a.ncol + col1];
+      }
+      double* dst = rb + eq * 4 * Ny;
+      dst[0 * Ny + row] = x0.x;
+      dst[1 * Ny + row] = x0.y;
+      dst[2 * Ny + row] = x1.x;
+      dst[3 * Ny + row] = x1.y;
+    }
+  } else {
+    // mean mode: U (vectors 0,1) and W (vectors 2,3) with the omega_y-type
+    // operator at k^2 = 0; forcing -dt/2 (3 L^n - L^{n-1}) (PF:403-404, 441-442)
+    for (int row = threadIdx.x; row < Ny; row += blockDim.x) {
+      const bool odd = row >= ne;
+      const int i = odd ? row - ne : row;
+      double acc[4] = {0, 0, 0, 0};
+      if (i >= 1) {
+        const int p = i, q = N - p;
+        for (int l = odd ? 1 : 0; l <= N; l += 2) {
+          const double t = a.t0[p * Ny + l] + c * a.t2[p * Ny + l];
+          const double2 U = a.au0[l], W = a.w0[l];
+          acc[0] += t * U.x; acc[1] += t * U.y; acc[2] += t * W.x; acc[3] += t * W.y;
+        }
+        const double2 f1p = a.c1n[p], f1q = a.c1n[q], g1p = a.c1p[p], g1q = a.c1p[q];
+        const double2 f3p = a.c3n[p], f3q = a.c3n[q], g3p = a.c3p[p], g3q = a.c3p[q];
+        const double hd = -1.0 * a.dt / 2.0;
+        const double2 Fp = make_double2(hd * (3.0 * f1p.x - g1p.x), hd * (3.0 * f1p.y - g1p.y));
+        const double2 Fq = make_double2(hd * (3.0 * f1q.x - g1q.x), hd * (3.0 * f1q.y - g1q.y));
+        const double2 Gp = make_double2(hd * (3.0 * f3p.x - g3p.x), hd * (3.0 * f3p.y - g3p.y));
+        const double2 Gq = make_double2(hd * (3.0 * f3q.x - g3q.x), hd * (3.0 * f3q.y - g3q.y));
+        const double s = odd ? -1.0 : 1.0;
+        if (p == q) {
+          acc[0] += Fp.x; acc[1] += Fp.y; acc[2] += Gp.x; acc[3] += Gp.y;
+        } else {
+          acc[0] += 0.5 * (Fp.x + s * Fq.x); acc[1] += 0.5 * (Fp.y + s * Fq.y);
+          acc[2] += 0.5 * (Gp.x + s * Gq.x); acc[3] += 0.5 * (Gp.y + s * Gq.y);
+        }
+      }
+      double* dst = rb + 4 * Ny;
+      for (int q4 = 0; q4 < 4; ++q4) dst[q4 * Ny + row] = acc[q4];
+    }
+  }
+  __syncthreads();
+  // ---- dense half-system inverses: x = Einv be, Oinv bo.  Thread = one row
+  // pair (2i, 2i+1) of one of the four half systems; 16-byte matrix loads.
+  const int neP = ne + (ne & 1), noP = no + (no & 1);
+  const int pe = neP / 2, po = noP / 2;           // row pairs per half system
+  for (int t = threadIdx.x; t < 2 * (pe + po); t += blockDim.x) {
+    const int eq = t / (pe + po);
+    if (mean && eq == 0) continue;
+    const int rr = t - eq * (pe + po);
+    const bool odd = rr >= pe;
+    const int i2 = 2 * (odd ? rr - pe : rr);
+    const int n = odd ? no : ne, nP = odd ? noP : neP;
+    const double* M = (eq ? a.invW : a.invV) + (long)g * a.stride + (odd ? (long)ne * neP : 0) + i2;
+    const double* rhs = rb + eq * 4 * Ny + (odd ? ne : 0);
+    double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+      double2 m4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) m4[u] = *reinterpret_cast<const double2*>(M + (long)(j + u) * nP);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double r = rhs[q * Ny + j + u];
+          x[q] = fma(m4[u].x, r, x[q]);
+          x[4 + q] = fma(m4[u].y, r, x[4 + q]);
+        }
+      }
+    }
+    for (; j < n; ++j) {
+      const double2 m = *reinterpret_cast<const double2*>(M + (long)j * nP);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double r = rhs[q * Ny + j];
+        x[q] = fma(m.x, r, x[q]);
+        x[4 + q] = fma(m.y, r, x[4 + q]);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = i2 + h;
+      if (i >= n) break;
+      const int l = 2 * i + (odd ? 1 : 0);
+      if (mean) {
+        a.au0[l] = make_double2(x[4 * h + 0], x[4 * h + 1]);
+        a.w0[l] = make_double2(x[4 * h + 2], x[4 * h + 3]);
+      } else {
+        double2* dst = eq ? a.oy : a.v;
+        dst[(long)l * a.ncol + col0] = make_double2(x[4 * h + 0], x[4 * h + 1]);
+        if (kzab > 0) dst[(long)l * a.ncol + col1] = make_double2(x[4 * h + 2], x[4 * h + 3]);
+      }
     }
   }
 }
 
-// One block per (kx, |kz|) group; modes m0 = (kz, kx), m1 = (Mz - kz, kx).
-__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
-  extern __shared__ double sm[];
-  const int Ny = a.Ny, N = a.N, ne = a.ne, no = a.no;
-  const int g = blockIdx.x;
-  const int kx = g % a.Mx, kzab = g / a.Mx;
-  const bool mean = (g == 0);
-  const bool two = (kzab > 0) || mean;
-  const int kz0 = kzab, kz1 = kzab ? a.Mz - kzab : 0;
-  const long col0 = (long)kz0 * a.Mx + kx, col1 = (long)kz1 * a.Mx + kx;
-  // LDS, 24 Ny doubles with aliasing by live range:
-  //   region A (8 Ny): rhs [2 eq][4 vec][Ny], then U, W [4 vec][Ny] each
-  //   region B (8 Ny): V, O (new coefficients)
-  //   region C (4 Ny): DV, then DW ; region D (4 Ny): DU
-  double* rb = sm;
-  double* su = sm;
-  double* sw = sm + 4 * Ny;
-  double* sv = sm + 8 * Ny;
-  double* so = sv + 4 * Ny;
-  double* sd1 = sm + 16 * Ny;
-  double* sd2 = sd1;
-  double* sd3 = sm + 20 * Ny;
-  const double c = a.dt * 0.5 / a.re;
+// ------------------------------------------------------------- kinematics
 
-  if (a.mode == 0) {
-    // ---- right-hand sides, parity-folded (rows of the half systems)
-    if (!mean) {
-      for (int idx = threadIdx.x; idx < 2 * Ny; idx += blockDim.x) {
-        const int eq = idx / Ny, row = idx - eq * Ny;
-        // row < ne: even system row; else odd system row (row - ne)
-        const bool odd = row >= ne;
-        const int i = odd ? row - ne : row;
-        const int nbc = eq ? 1 : 2;
-        double2 x0 = make_double2(0, 0), x1 = make_double2(0, 0);
-        if (i >= nbc) {
-          const int r = i - nbc;
-          const double2* src = eq ? (odd ? a.roO : a.roE) : (odd ? a.rvO : a.rvE);
-          x0 = src[(long)r * a.ncol + col0];
-          if (kzab > 0) x1 = src[(long)r * a.ncol + col1];
-        }
-        double* dst = rb + eq * 4 * Ny;
-        dst[0 * Ny + row] = x0.x;
-        dst[1 * Ny + row] = x0.y;
-        dst[2 * Ny + row] = x1.x;
-        dst[3 * Ny + row] = x1.y;
-      }
-    } else {
-      // mean mode: U (vectors 0,1) and W (vectors 2,3) with the omega_y-type
-      // operator at k^2 = 0; forcing -dt/2 (3 L^n - L^{n-1}) (PF:403-404, 441-442)
-      for (int row = threadIdx.x; row < Ny; row += blockDim.x) {
-        const bool odd = row >= ne;
-        const int i = odd ? row - ne : row;
-        double acc[4] = {0, 0, 0, 0};
-        if (i >= 1) {
-          const int p = i, q = N - p;
-          for (int l = odd ? 1 : 0; l <= N; l += 2) {
-            const double t = a.t0[p * Ny + l] + c * a.t2[p * Ny + l];
-            const double2 U = a.au0[l], W = a.w0[l];
-            acc[0] += t * U.x; acc[1] += t * U.y; acc[2] += t * W.x; acc[3] += t * W.y;
-          }
-          const double2 f1p = a.c1n[p], f1q = a.c1n[q], g1p = a.c1p[p], g1q = a.c1p[q];
-          const double2 f3p = a.c3n[p], f3q = a.c3n[q], g3p = a.c3p[p], g3q = a.c3p[q];
-          const double hd = -1.0 * a.dt / 2.0;
-          const double2 Fp = make_double2(hd * (3.0 * f1p.x - g1p.x), hd * (3.0 * f1p.y - g1p.y));
-          const double2 Fq = make_double2(hd * (3.0 * f1q.x - g1q.x), hd * (3.0 * f1q.y - g1q.y));
-          const double2 Gp = make_double2(hd * (3.0 * f3p.x - g3p.x), hd * (3.0 * f3p.y - g3p.y));
-          const double2 Gq = make_double2(hd * (3.0 * f3q.x - g3q.x), hd * (3.0 * f3q.y - g3q.y));
-          const double s = odd ? -1.0 : 1.0;
-          if (p == q) {
-            acc[0] += Fp.x; acc[1] += Fp.y; acc[2] += Gp.x; acc[3] += Gp.y;
-          } else {
-            acc[0] += 0.5 * (Fp.x + s * Fq.x); acc[1] += 0.5 * (Fp.y + s * Fq.y);
-            acc[2] += 0.5 * (Gp.x + s * Gq.x); acc[3] += 0.5 * (Gp.y + s * Gq.y);
-          }
-        }
-        double* dst = rb + 4 * Ny;
-        for (int q4 = 0; q4 < 4; ++q4) dst[q4 * Ny + row] = acc[q4];
-      }
-    }
-    __syncthreads();
-    // ---- dense half-system inverses: x = Einv be, Oinv bo.  Thread = one row
-    // pair (2i, 2i+1) of one of the four half systems; 16-byte matrix loads.
-    const int neP = ne + (ne & 1), noP = no + (no & 1);
-    const int pe = neP / 2, po = noP / 2;           // row pairs per half system
-    for (int t = threadIdx.x; t < 2 * (pe + po); t += blockDim.x) {
-      const int eq = t / (pe + po);
-      if (mean && eq == 0) continue;
-      const int rr = t - eq * (pe + po);
-      const bool odd = rr >= pe;
-      const int i2 = 2 * (odd ? rr - pe : rr);
-      const int n = odd ? no : ne, nP = odd ? noP : neP;
-      const double* M = (eq ? a.invW : a.invV) + (long)g * a.stride + (odd ? (long)ne * neP : 0) + i2;
-      const double* rhs = rb + eq * 4 * Ny + (odd ? ne : 0);
-      double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      int j = 0;
-      for (; j + 4 <= n; j += 4) {
-        double2 m4[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) m4[u] = *reinterpret_cast<const double2*>(M + (long)(j + u) * nP);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double r = rhs[q * Ny + j + u];
-            x[q] = fma(m4[u].x, r, x[q]);
-            x[4 + q] = fma(m4[u].y, r, x[4 + q]);
-          }
-        }
-      }
-      for (; j < n; ++j) {
-        const double2 m = *reinterpret_cast<const double2*>(M + (long)j * nP);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double r = rhs[q * Ny + j];
-          x[q] = fma(m.x, r, x[q]);
-          x[4 + q] = fma(m.y, r, x[4 + q]);
-        }
-      }
-      double* dst = eq ? so : sv;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int i = i2 + h;
-        if (i >= n) break;
-        const int l = 2 * i + (odd ? 1 : 0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) dst[q * Ny + l] = x[4 * h + q];
-      }
-    }
-    __syncthreads();
-  } else {
-    // kinematics only: fields from the current v, omega_y (or U, W for the mean)
-    for (int k = threadIdx.x; k < Ny; k += blockDim.x) {
-      if (mean) {
-        const double2 U = a.au0[k], W = a.w0[k];
-        so[0 * Ny + k] = U.x; so[1 * Ny + k] = U.y; so[2 * Ny + k] = W.x; so[3 * Ny + k] = W.y;
-      } else {
-        const double2 v0 = a.v[(long)k * a.ncol + col0];
-        const double2 o0 = a.oy[(long)k * a.ncol + col0];
-        double2 v1 = make_double2(0, 0), o1 = make_double2(0, 0);
-        if (two) {
-          v1 = a.v[(long)k * a.ncol + col1];
-          o1 = a.oy[(long)k * a.ncol + col1];
-        }
-        sv[0 * Ny + k] = v0.x; sv[1 * Ny + k] = v0.y; sv[2 * Ny + k] = v1.x; sv[3 * Ny + k] = v1.y;
-        so[0 * Ny + k] = o0.x; so[1 * Ny + k] = o0.y; so[2 * Ny + k] = o1.x; so[3 * Ny + k] = o1.y;
-      }
-    }
-    __syncthreads();
-  }
+// getvelocityandvorticity (PF:2347-2407) for every mode with k^2 != 0:
+//   u = (i alpha Dv - i beta omega_y)/k^2,  w = (i beta Dv + i alpha omega_y)/k^2,
+//   omega_x = Dw - i beta v,                omega_z = -Du + i alpha v,
+// and for the mean mode (PF:435-438, 469-472) u = U, w = W, omega_z = -DU,
+// omega_x = DW.  D is the coefficient-space derivative of dz (PF:570-599),
+// b_k = b_{k+2} + 2(k+1) a_{k+1}, b_0 halved.
+//
+// The recurrences split into two independent chains by degree parity: chain q
+// reads v at degrees of parity q and omega_y at parity 1-q, and produces
+// Dv, u, w at parity 1-q and Du, Dw, omega_x, omega_z at parity q.  One thread
+// runs one chain of one mode, backwards over the degree, with its loads issued
+// eight degrees ahead.
+struct KinArgs {
+  const double2 *v, *oy, *au0, *w0;
+  double2 *u, *w, *ox, *oz;
+  const double *al, *be;
+  int Mx, Mz, N, Ny;
+  long ncol;
+};
 
-  if (mean) {
-    // U in so[0..1], W in so[2..3]:  u00 = U, w00 = W, oz00 = -DU, ox00 = DW
-    lds_deriv(so, sd1, 4, N, Ny);
+__global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
+  const int q = blockIdx.y;  // chain: parity of the v degrees it reads
+  const int N = a.N;
+  if (blockIdx.x == gridDim.x - 1) {
+    // mean mode, one block: U, W staged in LDS, derivatives by lanes 0..3
+    if (q != 0) return;
+    extern __shared__ double sm[];
+    double* x = sm;              // U.re U.im W.re W.im   [4][Ny]
+    double* d = sm + 4 * a.Ny;   // DU.re DU.im DW.re DW.im
+    for (int k = threadIdx.x; k < a.Ny; k += blockDim.x) {
+      const double2 U = a.au0[k], W = a.w0[k];
+      x[0 * a.Ny + k] = U.x; x[1 * a.Ny + k] = U.y; x[2 * a.Ny + k] = W.x; x[3 * a.Ny + k] = W.y;
+    }
     __syncthreads();
-    for (int k = threadIdx.x; k < Ny; k += blockDim.x) {
-      const double2 U = make_double2(so[0 * Ny + k], so[1 * Ny + k]);
-      const double2 W = make_double2(so[2 * Ny + k], so[3 * Ny + k]);
-      const long off = (long)k * a.ncol;
-      if (a.mode == 0) {
-        a.au0[k] = U;
-        a.w0[k] = W;
-      }
-      a.u[off] = U;
-      a.w[off] = W;
-      a.oz[off] = make_double2(-1.0 * sd1[0 * Ny + k], -1.0 * sd1[1 * Ny + k]);
-      a.ox[off] = make_double2(sd1[2 * Ny + k], sd1[3 * Ny + k]);
-    }
-    return;
-  }
-
-  // ---- store v, omega_y and run the kinematics (getvelocityandvorticity)
-  const double al = a.al[kx];
-  const double be0 = a.be[kz0], be1 = a.be[kz1];
-  const double k2 = al * al + be0 * be0;
-  if (a.mode == 0) {
-    for (int k = threadIdx.x; k < Ny; k += blockDim.x) {
-      const long off = (long)k * a.ncol;
-      a.v[off + col0] = make_double2(sv[0 * Ny + k], sv[1 * Ny + k]);
-      a.oy[off + col0] = make_double2(so[0 * Ny + k], so[1 * Ny + k]);
-      if (kzab > 0) {
-        a.v[off + col1] = make_double2(sv[2 * Ny + k], sv[3 * Ny + k]);
-        a.oy[off + col1] = make_double2(so[2 * Ny + k], so[3 * Ny + k]);
-      }
-    }
-  }
-  lds_deriv(sv, sd1, 4, N, Ny);  // Dv
-  __syncthreads();
-  for (int k = threadIdx.x; k < Ny; k += blockDim.x) {
-    for (int m = 0; m < 2; ++m) {
-      const double be = m ? be1 : be0;
-      const double dvr = sd1[(2 * m) * Ny + k], dvi = sd1[(2 * m + 1) * Ny + k];
-      const double omr = so[(2 * m) * Ny + k], omi = so[(2 * m + 1) * Ny + k];
-      // PF:2368-2375: u = (i alpha Dv - i beta omega)/k^2, w = (i beta Dv + i alpha omega)/k^2
-      su[(2 * m + 1) * Ny + k] = al * dvr / k2 - be * omr / k2;
-      su[(2 * m) * Ny + k] = -1.0 * al * dvi / k2 + be * omi / k2;
-      sw[(2 * m + 1) * Ny + k] = be * dvr / k2 + al * omr / k2;
-      sw[(2 * m) * Ny + k] = -1.0 * dvi * be / k2 - 1.0 * omi * al / k2;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 64) lds_deriv(sw, sd2, 4, N, Ny);
-  else if (threadIdx.x >= 64 && threadIdx.x < 128) {
-    // second wave: Du (lane index shifted)
-    const int q = threadIdx.x - 64;
-    if (q < 4) {
-      const double* x = su + q * Ny;
-      double* b = sd3 + q * Ny;
+    if (threadIdx.x < 4) {
+      const double* xa = x + threadIdx.x * a.Ny;
+      double* b = d + threadIdx.x * a.Ny;
       double b2 = 0.0, b1 = 0.0;
       b[N] = 0.0;
       for (int k = N - 1; k >= 0; --k) {
-        const double bk = b2 + 2.0 * (k + 1) * x[k + 1];
+        const double bk = b2 + 2.0 * (k + 1) * xa[k + 1];
         b2 = b1;
         b1 = bk;
         b[k] = (k == 0) ? 0.5 * bk : bk;
       }
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < a.Ny; k += blockDim.x) {
+      const long off = (long)k * a.ncol;
+      a.u[off] = make_double2(x[0 * a.Ny + k], x[1 * a.Ny + k]);
+      a.w[off] = make_double2(x[2 * a.Ny + k], x[3 * a.Ny + k]);
+      a.oz[off] = make_double2(-1.0 * d[0 * a.Ny + k], -1.0 * d[1 * a.Ny + k]);
+      a.ox[off] = make_double2(d[2 * a.Ny + k], d[3 * a.Ny + k]);
+    }
+    return;
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < Ny; k += blockDim.x) {
-    const long off = (long)k * a.ncol;
-    for (int m = 0; m < 2; ++m) {
-      if (m == 1 && kzab == 0) break;
-      const long col = m ? col1 : col0;
-      const double be = m ? be1 : be0;
-      const double vr = sv[(2 * m) * Ny + k], vi = sv[(2 * m + 1) * Ny + k];
-      // PF:2388-2389: omega_x = Dw - i beta v ; PF:2401-2402: omega_z = -Du + i alpha v
-      const double2 oxv = make_double2(sd2[(2 * m) * Ny + k] + be * vi, sd2[(2 * m + 1) * Ny + k] - be * vr);
-      const double2 ozv = make_double2(-sd3[(2 * m) * Ny + k] - al * vi, -sd3[(2 * m + 1) * Ny + k] + al * vr);
-      a.u[off + col] = make_double2(su[(2 * m) * Ny + k], su[(2 * m + 1) * Ny + k]);
-      a.w[off + col] = make_double2(sw[(2 * m) * Ny + k], sw[(2 * m + 1) * Ny + k]);
-      a.ox[off + col] = oxv;
-      a.oz[off + col] = ozv;
+  const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cm >= a.ncol || cm == 0) return;
+  const int kz = (int)(cm / a.Mx);
+  const int kx = (int)(cm - (long)kz * a.Mx);
+  if (kz == a.Mz / 2) return;  // span Nyquist: never solved, stays zero (PF:247)
+  const double al = a.al[kx], be = a.be[kz];
+  const double k2 = al * al + be * be;
+  double2 vp = make_double2(0, 0);                    // v_{k+1}
+  double2 dv2 = make_double2(0, 0);                   // Dv_{k+2} (unhalved)
+  double2 up = make_double2(0, 0), wp = make_double2(0, 0);    // u_{k+1}, w_{k+1}
+  double2 du2 = make_double2(0, 0), dw2 = make_double2(0, 0);  // Du_{k+2}, Dw_{k+2}
+  for (int k0 = N; k0 >= 0; k0 -= 8) {
+    double2 in[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int k = k0 - s;
+      if (k >= 0) in[s] = (((k & 1) == q) ? a.v : a.oy)[(long)k * a.ncol + cm];
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int k = k0 - s;
+      if (k < 0) break;
+      const long off = (long)k * a.ncol + cm;
+      const double f = 2.0 * (k + 1);
+      if ((k & 1) == q) {
+        // Du_k, Dw_k from u_{k+1}, w_{k+1}; omega_x, omega_z with v_k
+        double2 du = make_double2(du2.x + f * up.x, du2.y + f * up.y);
+        double2 dw = make_double2(dw2.x + f * wp.x, dw2.y + f * wp.y);
+        du2 = du;
+        dw2 = dw;
+        if (k == 0) {
+          du.x *= 0.5; du.y *= 0.5; dw.x *= 0.5; dw.y *= 0.5;
+        }
+        const double2 vk = in[s];
+        a.ox[off] = make_double2(dw.x + be * vk.y, dw.y - be * vk.x);
+        a.oz[off] = make_double2(-du.x - al * vk.y, -du.y + al * vk.x);
+        vp = vk;
+      } else {
+        // Dv_k from v_{k+1}; u_k, w_k with omega_y_k
+        double2 dv = make_double2(dv2.x + f * vp.x, dv2.y + f * vp.y);
+        dv2 = dv;
+        if (k == 0) {
+          dv.x *= 0.5; dv.y *= 0.5;
+        }
+        const double omr = in[s].x, omi = in[s].y;
+        const double2 uk = make_double2(-1.0 * al * dv.y / k2 + be * omi / k2, al * dv.x / k2 - be * omr / k2);
+        const double2 wk = make_double2(-1.0 * dv.y * be / k2 - 1.0 * omi * al / k2, be * dv.x / k2 + al * omr / k2);
+        a.u[off] = uk;
+        a.w[off] = wk;
+        up = uk;
+        wp = wk;
+      }
     }
   }
 }
 
-hipError_t launch_solve(Ctx& c, int mode) {
+hipError_t launch_solve(Ctx& c) {
   const Dims& d = c.d;
   SolveArgs a{};
   a.rvE = c.rvE;
@@ -441,10 +431,6 @@ hipError_t launch_solve(Ctx& c, int mode) {
   a.invW = c.invW;
   a.v = c.v;
   a.oy = c.oy;
-  a.u = c.u;
-  a.w = c.w;
-  a.ox = c.ox;
-  a.oz = c.oz;
   a.au0 = c.au0;
   a.w0 = c.w0;
   a.c1n = c.cl1[c.hcur];
@@ -453,8 +439,6 @@ hipError_t launch_solve(Ctx& c, int mode) {
   a.c3p = c.cl3[1 - c.hcur];
   a.t0 = c.t0;
   a.t2 = c.t2;
-  a.al = c.alpha;
-  a.be = c.beta;
   a.Mx = d.Mx;
   a.Mz = d.Mz;
   a.N = d.N;
@@ -465,13 +449,41 @@ hipError_t launch_solve(Ctx& c, int mode) {
   a.ncol = d.ncol;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
-  a.mode = mode;
-  size_t shm = sizeof(double) * 24ull * d.Ny;
+  const size_t shm = sizeof(double) * 8ull * d.Ny;
   if (shm > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_solve, dim3(d.ngroups), dim3(256), shm, c.stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_kinematics(Ctx& c) {
+  const Dims& d = c.d;
+  KinArgs a{};
+  a.v = c.v;
+  a.oy = c.oy;
+  a.au0 = c.au0;
+  a.w0 = c.w0;
+  a.u = c.u;
+  a.w = c.w;
+  a.ox = c.ox;
+  a.oz = c.oz;
+  a.al = c.alpha;
+  a.be = c.beta;
+  a.Mx = d.Mx;
+  a.Mz = d.Mz;
+  a.N = d.N;
+  a.Ny = d.Ny;
+  a.ncol = d.ncol;
+  const unsigned nb = (unsigned)((d.ncol + 255) / 256) + 1;  // + the mean-mode block
+  const size_t shm = sizeof(double) * 8ull * d.Ny;
+  if (shm > 65536) {
+    hipError_t e =
+        hipFuncSetAttribute((const void*)k_kinematics, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_kinematics, dim3(nb, 2), dim3(256), shm, c.stream, a);
   return hipGetLastError();
 }
 
