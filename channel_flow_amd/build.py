@@ -1,6 +1,10 @@
-"""Builds libchf.so (HIP, gfx950) in-tree: channel_flow_amd/lib/libchf.so.
+"""Builds, in-tree, libchf.so (HIP, gfx950) and the Fortran driver that calls
+it through ISO_C_BINDING:
 
-Usage: python -m channel_flow_amd.build [--jobs N]
+    channel_flow_amd/lib/libchf.so
+    channel_flow_amd/lib/poiseuilleflow_gpu     (amdflang; rpath $ORIGIN)
+
+Usage: python -m channel_flow_amd.build [--jobs N] [--no-fortran]
 """
 from __future__ import annotations
 
@@ -19,6 +23,9 @@ OBJDIR = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(LIBDIR, "libchf.so")
 
 SOURCES = ["api.hip", "fft.hip", "cheb.hip", "solve.hip"]
+FORTRAN_DIR = os.path.join(PKG, "fortran")
+FORTRAN_SOURCES = ["chf_binding.f90", "paraandcons_gpu.f90", "poiseuilleflow_gpu.f90"]  # in module order
+DRIVER = os.path.join(LIBDIR, "poiseuilleflow_gpu")
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -45,7 +52,41 @@ def _compile(src):
     return obj
 
 
-def build(jobs: int = 4, verbose: bool = False) -> str:
+def flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/bin/amdflang", shutil.which("amdflang"),
+                 shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compiles the Fortran driver against the freshly built libchf.so."""
+    fc = flang()
+    if fc is None:
+        raise RuntimeError("no Fortran compiler (amdflang) found")
+    moddir = os.path.join(ROOT, "_build", "mod")
+    os.makedirs(moddir, exist_ok=True)
+    objs = []
+    for src in FORTRAN_SOURCES:
+        obj = os.path.join(OBJDIR, src + ".o")
+        cmd = [fc, "-O2", "-J" + moddir, "-c", os.path.join(FORTRAN_DIR, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    tmp = DRIVER + ".tmp"
+    cmd = [fc, "-o", tmp] + objs + ["-L" + LIBDIR, "-lchf", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
+
+
+def build(jobs: int = 4, verbose: bool = False, fortran: bool = True) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -58,14 +99,17 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    if fortran:
+        build_fortran(verbose)
     return LIB
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--no-fortran", action="store_true")
     a = ap.parse_args(argv)
-    build(a.jobs, verbose=True)
+    build(a.jobs, verbose=True, fortran=not a.no_fortran)
 
 
 if __name__ == "__main__":

@@ -41,8 +41,8 @@ struct SynthP {
   int Ny, P;
   long ncold;
   __device__ void select(int z) {
-    in = inf[z];
-    out = outf[z];
+    in = pick6(inf, z);
+    out = pick6(outf, z);
   }
   __device__ int aindex(int m, int row, int k0, int n4_) const {
     return (int)(((long)f * row * (2 * k0 + m)) % n4_);
@@ -126,8 +126,8 @@ struct AnalP {
   long ncold;
   double inv2n;
   __device__ void select(int z) {
-    in = inf[z];
-    out = outf[z];
+    in = pick6(inf, z);
+    out = pick6(outf, z);
   }
   __device__ int aindex(int m, int row, int k0, int n4_) const {
     return (int)(((long)(2 * row + m) * k0) % n4_);

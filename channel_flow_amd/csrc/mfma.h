@@ -39,6 +39,21 @@ struct Mf<float> {
   static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
+// One of six per-field values, chosen by a wave-uniform index through
+// constant-index reads: indexing the array with the runtime value would make
+// the parameter block addressable and move it to scratch memory.
+template <class T>
+__device__ __forceinline__ T pick6(T const (&a)[6], int z) {
+  switch (z) {
+    case 0: return a[0];
+    case 1: return a[1];
+    case 2: return a[2];
+    case 3: return a[3];
+    case 4: return a[4];
+    default: return a[5];
+  }
+}
+
 template <class P, int RBW, int CB>
 __global__ __launch_bounds__(256) void k_mfma(P pb) {
   constexpr int NA = P::NACC, NB = P::NB, KC = 16, BN = 16 * CB;

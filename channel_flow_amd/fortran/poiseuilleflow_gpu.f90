@@ -1,0 +1,144 @@
+! Time loop of poiseuilleflow (PF:220-532) on one MI355X through libchf.so.
+!
+!   poiseuilleflow_gpu [parameters.nml]
+!
+! The loop keeps the reference's order: the first iteration only solves (the
+! state read by `initial` already holds h^n, PF:222); every later iteration
+! runs getnonlinear, writes the restart record every `interupt` steps
+! (output, PF:2460-2504) and then the per-mode and mean-mode solves
+! (PF:242-474).  The restart files use the reference's own record layout and
+! name, data00000002_instant.000.dat (PF:2466-2480), so either program can
+! continue the other's run.
+program poiseuilleflow_gpu
+  use, intrinsic :: iso_c_binding
+  use paraandcons
+  use chf_binding
+  implicit none
+
+  type(chf_params) :: prm
+  type(chf_state) :: st
+  type(c_ptr) :: ctx
+  character(len=512) :: arg
+  integer :: istep, nsolve
+  integer(8) :: c0, c1, crate
+  real*8 :: secs
+
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, arg)
+    call read_parameters(trim(arg))
+  end if
+  call allocate_state()
+
+  prm%nnx = nnx; prm%nny = nny; prm%nnz = nnz
+  prm%aphi = aphi; prm%beta = beta; prm%dt = dt; prm%re = re; prm%re_p = re_p
+  prm%precision = precision
+  ctx = c_null_ptr
+  call chf_check(chf_create(prm, ctx), ctx, 'chf_create')
+
+  st%clamb1n = c_loc(clamb1n); st%clamb1nm1 = c_loc(clamb1nm1)
+  st%clamb3n = c_loc(clamb3n); st%clamb3nm1 = c_loc(clamb3nm1)
+  st%au0 = c_loc(au0); st%w0 = c_loc(w0)
+  st%vnp1 = c_loc(vnp1); st%omegaynp1 = c_loc(omegaynp1)
+  st%rhsvn = c_loc(rhsvn); st%rhsomegayn = c_loc(rhsomegayn)
+  st%unp1 = c_loc(unp1); st%wnp1 = c_loc(wnp1)
+
+  if (isign == 1) then
+    call read_restart(trim(restart_in))
+    call chf_check(chf_set_state(ctx, st), ctx, 'chf_set_state')
+  else
+    call laminar_start()
+  end if
+  write(*, '(a,i0,a,i0,a,i0,a,f8.3,a)') 'channel ', nnx, ' x ', nnz + 1, ' x ', nny, &
+      ', device ', dble(chf_device_bytes(ctx)) / 1.0d9, ' GB'
+
+  nsolve = 0
+  call system_clock(c0, crate)
+  do istep = istepbegin, nstep                  ! PF:220
+    if (istep /= istepbegin) then
+      call chf_check(chf_nonlinear(ctx), ctx, 'chf_nonlinear')   ! PF:224
+      if (mod(istep, interupt) == 0) then        ! output, PF:225 / 2465
+        call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
+        call write_restart(istep)
+      end if
+      if (mod(istep, outputstep) == 0) then
+        if (mod(istep, interupt) /= 0) call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
+        call mean_statistics(istep)
+      end if
+    end if
+    call chf_check(chf_solve(ctx), ctx, 'chf_solve')            ! PF:242-474
+    nsolve = nsolve + 1
+  end do
+  call chf_check(chf_sync(ctx), ctx, 'chf_sync')
+  call system_clock(c1)
+  secs = dble(c1 - c0) / dble(crate)
+  write(*, '(a,i0,a,f10.4,a,f10.3,a)') 'steps ', nsolve, ', wall ', secs, ' s, ', &
+      dble(nsolve) / max(secs, 1.0d-12), ' steps/s'
+  call chf_check(chf_destroy(ctx), c_null_ptr, 'chf_destroy')
+
+contains
+
+  ! initial(isign=1), PF:1861-1924: two unformatted records
+  subroutine read_restart(fname)
+    character(len=*), intent(in) :: fname
+    integer :: u, ios
+    open(newunit=u, file=fname, status='old', form='unformatted', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(*, '(a,a)') 'cannot open restart file ', fname
+      error stop 1
+    end if
+    read(u) clamb1n, clamb1nm1, clamb3n, clamb3nm1, au0, w0
+    read(u) vnp1, omegaynp1, rhsvn, rhsomegayn, unp1, wnp1
+    close(u)
+    write(*, '(a,a)') 'read initial data from ', fname
+  end subroutine read_restart
+
+  ! output, PF:2460-2504
+  subroutine write_restart(istep)
+    integer, intent(in) :: istep
+    character(len=600) :: fname
+    integer :: u
+    write(fname, '(a,a,i8.8,a,i3.3,a)') trim(output_dir), '/data', 2, '_instant.', 0, '.dat'
+    open(newunit=u, file=trim(fname), status='replace', form='unformatted')
+    write(u) clamb1n, clamb1nm1, clamb3n, clamb3nm1, au0, w0
+    write(u) vnp1, omegaynp1, rhsvn, rhsomegayn, unp1, wnp1
+    close(u)
+    write(*, '(a,i0,a,a)') 'step ', istep, ': restart record written to ', trim(fname)
+  end subroutine write_restart
+
+  ! Laminar start U = (re re_p / 2)(1 - y^2), v = omega_y = 0: the exact
+  ! steady state of the scheme (Chebyshev a0 = U_c/2, a2 = -U_c/2).
+  subroutine laminar_start()
+    real*8 :: uc
+    uc = 0.5d0 * re * re_p
+    au0 = 0.0d0
+    w0 = 0.0d0
+    au0(1, 0) = 0.5d0 * uc
+    au0(1, 2) = -0.5d0 * uc
+    vnp1 = 0.0d0
+    omegaynp1 = 0.0d0
+    call chf_check(chf_init_fields(ctx, vnp1, omegaynp1, au0, w0), ctx, 'chf_init_fields')
+    call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
+  end subroutine laminar_start
+
+  ! Mean-flow quantities of statistics (PF:3170-3190) from the mean mode:
+  ! wall shear dU/dy(+-1) = sum_k a_k k^2 (+-1)^(k+1), bulk velocity
+  ! (1/2) int U dy = sum_{k even} a_k / (1 - k^2).
+  subroutine mean_statistics(istep)
+    integer, intent(in) :: istep
+    real*8 :: tau_top, tau_bot, ub, s
+    integer :: k
+    tau_top = 0.0d0
+    tau_bot = 0.0d0
+    ub = 0.0d0
+    do k = 0, nz
+      s = merge(1.0d0, -1.0d0, mod(k + 1, 2) == 0)
+      tau_top = tau_top + au0(1, k) * dble(k) ** 2
+      tau_bot = tau_bot + au0(1, k) * dble(k) ** 2 * s
+      if (mod(k, 2) == 0) ub = ub + au0(1, k) / (1.0d0 - dble(k) ** 2)
+    end do
+    write(*, '(a,i8,a,es14.6,a,es14.6,a,es14.6,a,es14.6)') 'istep ', istep, &
+        '  dU/dy(+1) ', tau_top, '  dU/dy(-1) ', tau_bot, '  U_bulk ', ub, &
+        '  -dp/dx ', re_p
+  end subroutine mean_statistics
+
+end program poiseuilleflow_gpu
