@@ -94,6 +94,18 @@ class Channel:
         self._check(self._lib.chf_get_state(self._ctx, ctypes.byref(cs)), "chf_get_state")
         return out
 
+    def load_restart(self, path: str):
+        """initial(isign=1) (PF:1861-1924): state from a restart file in the
+        reference's two-record layout."""
+        from . import restart
+        p = self.params
+        self.set_state(restart.read_restart(path, p.nnx, p.nny, p.nnz))
+
+    def save_restart(self, path: str):
+        """output (PF:2460-2504): the pre-solve state as a restart file."""
+        from . import restart
+        restart.write_restart(path, self.get_state())
+
     def initial(self, vnp1, omegaynp1, au0, w0):
         """Cold start from v, omega_y and the mean profiles (PF:1932-1946)."""
         v = self._as_field(vnp1, "vnp1")
