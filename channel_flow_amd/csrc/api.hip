@@ -148,7 +148,7 @@ void free_all(Ctx* c) {
   f(c->hvc); f(c->hgc);
   f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
-  f(c->invV); f(c->invW); f(c->twx); f(c->twz); f(c->costab);
+  f(c->inv); f(c->twx); f(c->twz); f(c->costab);
   f(c->gv); f(c->go);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
@@ -171,7 +171,8 @@ int build(Ctx* c) {
   d.Pn = d.N / 2 + 1;
   d.ncol = (long)d.Mz * d.Mx;
   d.ngroups = d.Mx * (d.Mz / 2);
-  d.inv_stride = (long)d.ne * (d.ne + (d.ne & 1)) + (long)d.no * (d.no + (d.no & 1));
+  d.inv_row = 2L * ((d.ne + (d.ne & 1)) + (d.no + (d.no & 1)));
+  d.inv_stride = (long)d.ne * d.inv_row;
 
   CHF_TRY(hipGetDevice(&c->device));
   int max_lds = 0;
@@ -221,8 +222,7 @@ int build(Ctx* c) {
   CHF_TRY(dalloc(c, &c->roO, (size_t)d.ne * d.ncol));
   CHF_TRY(dalloc(c, &c->gv, fld));
   CHF_TRY(dalloc(c, &c->go, fld));
-  CHF_TRY(dalloc(c, &c->invV, (size_t)d.ngroups * d.inv_stride));
-  CHF_TRY(dalloc(c, &c->invW, (size_t)d.ngroups * d.inv_stride));
+  CHF_TRY(dalloc(c, &c->inv, (size_t)d.ngroups * d.inv_stride));
 
   // wavenumbers (PF:123-134, id = 0)
   std::vector<double> al(d.Mx), be(d.Mz);

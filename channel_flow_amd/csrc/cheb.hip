@@ -29,8 +29,10 @@ template <class TI, class TO, class TCc>
 struct SynthP {
   using TC = TCc;
   static constexpr int NACC = 2, NB = 2;
+  static constexpr bool kRecurrence = false;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
+  const double* rtab;  // FP64 cosines for the recurrence form
   int n4;        // 4N
   int f;         // grid factor
   MfmaShape sh;  // M = folded points, K = ne
@@ -65,6 +67,7 @@ static hipError_t synth_t(Ctx& c, const TC* costab, int f, int Ph, int P, const 
                           TO* const* out, int nf) {
   SynthP<TI, TO, TC> pb{};
   pb.costab = costab;
+  pb.rtab = c.costab;
   pb.n4 = 4 * c.d.N;
   pb.f = f;
   pb.sh = mfma_shape(Ph, c.d.ne);
@@ -114,8 +117,10 @@ template <class TI, class TO, class TCc>
 struct AnalP {
   using TC = TCc;
   static constexpr int NACC = 2, NB = 2;
+  static constexpr bool kRecurrence = true;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
+  const double* rtab;  // FP64 cosines for the recurrence form
   int n4;
   MfmaShape sh;  // M = ne, K = Ph
   const TI* inf[6];
@@ -157,6 +162,7 @@ template <class TI, class TC>
 static hipError_t analysis_t(Ctx& c, const TC* costab, const TI* const* in, double2* const* out, int nf) {
   AnalP<TI, double, TC> pb{};
   pb.costab = costab;
+  pb.rtab = c.costab;
   pb.n4 = 4 * c.d.N;
   pb.sh = mfma_shape(c.d.ne, c.d.Ph);
   for (int i = 0; i < nf; ++i) {
@@ -260,6 +266,7 @@ template <int POFF>
 struct RhsSynP {
   using TC = double;
   static constexpr int NACC = 2, NB = 2;
+  static constexpr bool kRecurrence = false;
   static constexpr int bsel(int m) { return m; }
   const double* costab;
   int n4;

@@ -25,7 +25,8 @@ struct Dims {
   int Pn;     // N+1-point grid folded by parity: N/2+1
   long ncol;  // complex columns per plane: Mz*Mx
   int ngroups;  // (kx,|kz|) operator groups: Mx*(Mz/2)
-  long inv_stride;  // doubles per group per equation: ne*neP + no*noP (even leading dims)
+  long inv_row;     // doubles per inverse row j: 2 (neP + noP), the four half systems side by side
+  long inv_stride;  // doubles per group: ne * inv_row
 };
 
 // Device-side view of the wall-normal matrices (all point-major, row = point).
@@ -94,8 +95,12 @@ struct Ctx {
   double* costab = nullptr;  // cos(pi n / 2N), n < 4N
   float* costabf = nullptr;  // the same rounded to FP32 (mixed form)
   float2 *twxf = nullptr, *twzf = nullptr;  // FP32 twiddles (mixed form)
-  double* invV = nullptr;  // [ngroups][ne^2 + no^2] transposed inverses (v)
-  double* invW = nullptr;  // [ngroups][ne^2 + no^2] (omega_y; group 0 = mean mode)
+  // Half-system inverses, transposed: group g, row j (< ne) holds
+  //   [v even (neP)][v odd (noP)][omega even (neP)][omega odd (noP)]
+  // entries i of column j, zero-padded (i >= n, and j >= no for the odd halves),
+  // so a solve block reads one contiguous row per step.  Group 0: mean mode
+  // (omega_y-type operator at k^2 = 0), its v halves are zero.
+  double* inv = nullptr;
   double2* twx = nullptr;  // exp(-2 pi i m/Lx), m < Lx
   double2* twz = nullptr;  // exp(-2 pi i m/Lz), m < Lz
   long long bytes = 0;

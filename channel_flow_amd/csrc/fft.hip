@@ -429,6 +429,152 @@ __global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(RegX<V> a) 
   }
 }
 
+// LDS slot of complex element a in a Stockham buffer: one spare slot every 16
+// elements spreads the stride-R writes of the early stages over the banks.
+__device__ __forceinline__ int spad(int a) { return a + (a >> 4); }
+
+// One Stockham autosort radix-R stage over a line of L points held in LDS
+// (Ns = product of the radices already applied; thread j < L/R):
+//   v[r] = in[j + r L/R] * W_{Ns R}^{r (j mod Ns)},  v = DFT_R(v),
+//   out[(j / Ns) Ns R + (j mod Ns) + r Ns] = v[r].
+// After the last stage the line is in natural order.  `tw` = e^{-2 pi i m / L}.
+template <int L, int R, int Ns, int S, class V>
+__device__ __forceinline__ void st_stage(const V* in, V* out, const V* tw, int j) {
+  if (j >= L / R) return;
+  V v[R];
+  const int k = j % Ns;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = in[spad(j + r * (L / R))];
+  if (Ns > 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      V w = tw[r * k * (L / (Ns * R))];
+      if (S > 0) w.y = -w.y;
+      v[r] = cmulv(v[r], w);
+    }
+  }
+  Dft<R, S, V>::run(v);
+  const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[spad(base + r * Ns)] = v[r];
+}
+
+// Streamwise pass, one 64-lane wavefront per (y point, span point) line, for
+// L = RA RB RC with L / RC = 64.  The inverse transforms run the stages
+// RA, RB, RC; the last one leaves lane j holding the physical points j + 64 r
+// (r < RC) of every field, so the six products form the Lamb vector in
+// registers, and the forward transforms start with a radix-RC stage on exactly
+// those points (Ns = 1, no twiddle) before RB and RA.  Fields are packed two
+// per complex transform as in k_xpass_reg; block = NL lines.
+template <int L, int RA, int RB, int RC, int NL, class V>
+__global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
+  static_assert(L == RA * RB * RC && L / RC == 64, "one wavefront per line");
+  using T = decltype(V{}.x);
+  constexpr int PL = L + (L >> 4) + 1;  // padded buffer length
+  extern __shared__ __align__(16) unsigned char smraw[];
+  V* sm = reinterpret_cast<V*>(smraw);
+  V* tw = sm;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  V* b0 = sm + L + wave * 2 * PL;
+  V* b1 = b0 + PL;
+  const long line = (long)blockIdx.x * NL + wave;
+  const bool valid = line < a.nlines;
+  const int Mx = a.Mx;
+  for (int i = threadIdx.x; i < L; i += 64 * NL) tw[i] = a.tw[i];
+  V r[3][RC];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (valid) {
+      // natural-order spectrum: F1 + i F2 at m, conj(F1) + i conj(F2) at L - m,
+      // zero on the padding band Mx <= m <= L - Mx; Re of the kx = 0 entry (PF:1116-1118)
+      const V* f1 = a.in[2 * q] + line * Mx;
+      const V* f2 = a.in[2 * q + 1] + line * Mx;
+      for (int m = lane; m < Mx; m += 64) {
+        const V A = f1[m], B = f2[m];
+        if (m == 0) {
+          b0[spad(0)] = C2<T>::make(A.x, B.x);
+        } else {
+          b0[spad(m)] = C2<T>::make(A.x - B.y, A.y + B.x);
+          b0[spad(L - m)] = C2<T>::make(A.x + B.y, B.x - A.y);
+        }
+      }
+      for (int m = Mx + lane; m <= L - Mx; m += 64) b0[spad(m)] = C2<T>::make(0, 0);
+    }
+    __syncthreads();
+    st_stage<L, RA, 1, +1>(b0, b1, tw, lane);
+    __syncthreads();
+    st_stage<L, RB, RA, +1>(b1, b0, tw, lane);
+    __syncthreads();
+    {
+      // last inverse stage (Ns = 64): lane j reads j + 64 r, keeps the output in registers
+#pragma unroll
+      for (int rr = 0; rr < RC; ++rr) r[q][rr] = b0[spad(lane + 64 * rr)];
+#pragma unroll
+      for (int rr = 1; rr < RC; ++rr) {
+        V w = tw[rr * lane];
+        w.y = -w.y;
+        r[q][rr] = cmulv(r[q][rr], w);
+      }
+      Dft<RC, +1, V>::run(r[q]);
+    }
+    __syncthreads();
+  }
+  // Lamb vector at x = lane + 64 r: (L_x + i L_y) into r[0], (L_z + 0i) into r[1]
+  const T rep = (T)a.re_p;
+#pragma unroll
+  for (int i = 0; i < RC; ++i) {
+    const T uu = r[0][i].x, vv = r[0][i].y, ww = r[1][i].x, oxx = r[1][i].y;
+    const T oyy = r[2][i].x, ozz = r[2][i].y;
+    const T lx = (oyy * ww - rep) - ozz * vv;
+    const T ly = ozz * uu - oxx * ww;
+    const T lz = oxx * vv - oyy * uu;
+    r[0][i] = C2<T>::make(lx, ly);
+    r[1][i] = C2<T>::make(lz, 0);
+  }
+  const T sc = (T)a.scale;
+  const T half = (T)0.5;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    // first forward stage (R = RC, Ns = 1) on the registers
+    Dft<RC, -1, V>::run(r[q]);
+#pragma unroll
+    for (int rr = 0; rr < RC; ++rr) b0[spad(lane * RC + rr)] = r[q][rr];
+    __syncthreads();
+    st_stage<L, RB, RC, -1>(b0, b1, tw, lane);
+    __syncthreads();
+    st_stage<L, RA, RC * RB, -1>(b1, b0, tw, lane);
+    __syncthreads();
+    if (valid) {
+      if (q == 0) {
+        // separate the packed pair: F1 = (G(m) + conj G(-m))/2, F2 = (G(m) - conj G(-m))/(2i)
+        for (int m = lane; m < Mx; m += 64) {
+          const V Gp = b0[spad(m)];
+          const V Gc = b0[spad(m == 0 ? 0 : L - m)];
+          a.out[0][line * Mx + m] = C2<T>::make(half * (Gp.x + Gc.x) * sc, half * (Gp.y - Gc.y) * sc);
+          a.out[1][line * Mx + m] = C2<T>::make(half * (Gp.y + Gc.y) * sc, -half * (Gp.x - Gc.x) * sc);
+        }
+      } else {
+        for (int m = lane; m < Mx; m += 64) {
+          const V g = b0[spad(m)];
+          a.out[2][line * Mx + m] = C2<T>::make(g.x * sc, g.y * sc);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int L, int RA, int RB, int RC, class V>
+static hipError_t run_xpass_st(Ctx& c, const RegX<V>& a) {
+  constexpr int NL = sizeof(V) == 16 ? 2 : 4;  // lines per block
+  constexpr int PL = L + (L >> 4) + 1;
+  const long blocks = (a.nlines + NL - 1) / NL;
+  const size_t shm = sizeof(V) * (L + (size_t)NL * 2 * PL);
+  hipLaunchKernelGGL((k_xpass_st<L, RA, RB, RC, NL, V>), dim3((unsigned)blocks), dim3(64 * NL), shm,
+                     c.stream, a);
+  return hipGetLastError();
+}
+
 // Span passes: a group handles TX span lines (consecutive kx) of one plane;
 // thread (t, tx) = position t of line tx, tx fastest for coalesced access.
 template <int N1, int N2, int TX, bool INV, class V>
@@ -513,8 +659,8 @@ static hipError_t xpass_reg(Ctx& c, const RegX<V>& a) {
     case 48: return run_xpass_reg<6, 8>(c, a);
     case 96: return run_xpass_reg<8, 12>(c, a);
     case 144: return run_xpass_reg<12, 12>(c, a);
-    case 192: return run_xpass_reg<12, 16>(c, a);
-    case 384: return run_xpass_reg<16, 24>(c, a);
+    case 192: return run_xpass_st<192, 8, 8, 3>(c, a);
+    case 384: return run_xpass_st<384, 8, 8, 6>(c, a);
     default: return hipErrorInvalidValue;
   }
 }

@@ -39,6 +39,13 @@ struct Mf<float> {
   static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
+// A contraction with P::kRecurrence set and FP32 arithmetic takes its A
+// operand from an FP64 cosine recurrence in registers (one FMA per matrix op,
+// no LDS table); the others read the LDS table.  Which is faster depends on
+// the shape (measured: recurrence for the analysis, table for the synthesis).
+template <class P>
+constexpr bool cos_recurrence() { return sizeof(typename P::TC) == 4 && P::kRecurrence; }
+
 // One of six per-field values, chosen by a wave-uniform index through
 // constant-index reads: indexing the array with the runtime value would make
 // the parameter block addressable and move it to scratch memory.
@@ -149,18 +156,27 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
   constexpr int NA = P::NACC, NB = P::NB, KC = 16, BN = 16 * CB;
   constexpr int SB = (BN % 32 == 0) ? BN + 16 : BN;
   constexpr int PER = (KC * BN + 255) / 256;  // tile elements per thread
+  constexpr bool REC = cos_recurrence<P>();
   __shared__ TC sB[NB][KC][SB];
   extern __shared__ __align__(16) unsigned char stabraw[];
-  TC* stab = reinterpret_cast<TC*>(stabraw);  // 4N cosines
+  TC* stab = reinterpret_cast<TC*>(stabraw);  // 4N cosines (table form)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   P p = pb;
   p.select(blockIdx.z);
   const int n4 = p.n4;
-  for (int i = threadIdx.x; i < n4; i += 256) stab[i] = p.costab[i];
-  const long col0 = (long)blockIdx.x * BN;
+  if constexpr (!REC)
+    for (int i = threadIdx.x; i < n4; i += 256) stab[i] = p.costab[i];
+  // Column tiles in XCD-contiguous order: the 16-column tiles that share the
+  // cache lines of narrow (FP32) rows are written through the same L2.
+  const long tile = (gridDim.x & 7) ? (long)blockIdx.x
+                                    : (long)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+  const long col0 = tile * BN;
   const int MB = p.sh.MB, KB = p.sh.KB;
-  // per (m, r): current table index and its per-k-block increment
+  // Table form, per (m, r): current table index and its per-k-block increment.
+  // Recurrence form: the lane's A values along k are cos(phi0 + n dphi), so
+  // c_{n+1} = 2 cos(dphi) c_n - c_{n-1}, run in FP64 from two table values.
   int idx[NA][RBW], stp[NA][RBW];
+  double cc[NA][RBW], cp[NA][RBW], c2[NA][RBW];
 #pragma unroll
   for (int m = 0; m < NA; ++m)
 #pragma unroll
@@ -169,6 +185,14 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
       const int k0 = lane >> 4;
       idx[m][r] = p.aindex(m, row, k0, n4);
       stp[m][r] = p.astep(m, row, n4);  // index advance for k -> k + 4
+      if constexpr (REC) {
+        const int i0 = idx[m][r], st = stp[m][r];
+        int im = i0 - st;
+        im = im < 0 ? im + n4 : im;
+        cc[m][r] = p.rtab[i0];
+        cp[m][r] = p.rtab[im];
+        c2[m][r] = 2.0 * p.rtab[st];
+      }
     }
   typename MF::acc acc[NA][RBW][CB];
 #pragma unroll
@@ -222,9 +246,17 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
           if (rb < MB) {
 #pragma unroll
             for (int m = 0; m < NA; ++m) {
-              const TC a = stab[idx[m][r]];
-              int t = idx[m][r] + stp[m][r];
-              idx[m][r] = t >= n4 ? t - n4 : t;
+              TC a;
+              if constexpr (REC) {
+                a = (TC)cc[m][r];
+                const double nx = fma(c2[m][r], cc[m][r], -cp[m][r]);
+                cp[m][r] = cc[m][r];
+                cc[m][r] = nx;
+              } else {
+                a = stab[idx[m][r]];
+                int t = idx[m][r] + stp[m][r];
+                idx[m][r] = t >= n4 ? t - n4 : t;
+              }
 #pragma unroll
               for (int c = 0; c < CB; ++c)
                 acc[m][r][c] = MF::mma(a, bv[P::bsel(m)][c], acc[m][r][c]);
@@ -262,8 +294,8 @@ hipError_t run_mfma_cos_rbw(hipStream_t st, const P& p, int nz) {
   constexpr int budget = 8 / (P::NACC * RBW) > 0 ? 8 / (P::NACC * RBW) : 1;
   constexpr int CB = budget >= 4 ? 4 : (budget >= 2 ? 2 : 1);
   const long tiles = (p.ncold + 16 * CB - 1) / (16 * CB);
-  hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB>), dim3((unsigned)tiles, 1, nz), dim3(256),
-                     sizeof(typename P::TC) * p.n4, st, p);
+  const size_t shm = cos_recurrence<P>() ? 0 : sizeof(typename P::TC) * p.n4;
+  hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB>), dim3((unsigned)tiles, 1, nz), dim3(256), shm, st, p);
   return hipGetLastError();
 }
 

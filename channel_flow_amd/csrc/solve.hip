@@ -22,9 +22,9 @@ namespace chf {
 struct SetupArgs {
   const double *t0, *t2, *t4;
   const double *al, *be;
-  double *invV, *invW;
+  double* inv;
   int Mx, N, Ny, ne, no;
-  long stride;
+  long stride, row;
   double dt, re;
 };
 
@@ -51,11 +51,12 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
   const int kx = g % a.Mx, kz = g / a.Mx;
   const double k2 = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
   const int n = par ? a.no : a.ne;
-  const int nP = n + (n & 1);  // even leading dimension: 16-byte aligned row pairs
-  const int neP = a.ne + (a.ne & 1);
-  double* out = (eq ? a.invW : a.invV) + (long)g * a.stride + (par ? (long)a.ne * neP : 0);
+  const int nP = n + (n & 1);  // even width: 16-byte aligned row pairs
+  const int neP = a.ne + (a.ne & 1), noP = a.no + (a.no & 1);
+  double* out = a.inv + (long)g * a.stride + eq * (neP + noP) + par * neP;
   if (eq == 0 && g == 0) {  // k^2 = 0 has no v system (PF:184, 247)
-    for (int i = threadIdx.x; i < n * nP; i += blockDim.x) out[i] = 0.0;
+    for (int idx = threadIdx.x; idx < a.ne * nP; idx += blockDim.x)
+      out[(long)(idx / nP) * a.row + idx % nP] = 0.0;
     return;
   }
   double* A = sm;  // n x n row-major
@@ -125,10 +126,10 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
     }
     __syncthreads();
   }
-  // transposed store: out[j*nP + i] = Ainv[i][j], zero in the padding row i = n
-  for (int idx = threadIdx.x; idx < n * nP; idx += blockDim.x) {
+  // transposed store: row j, entry i = Ainv[i][j]; zero padding for i >= n and j >= n
+  for (int idx = threadIdx.x; idx < a.ne * nP; idx += blockDim.x) {
     const int j = idx / nP, i = idx - j * nP;
-    out[idx] = (i < n) ? A[i * n + j] : 0.0;
+    out[(long)j * a.row + i] = (i < n && j < n) ? A[i * n + j] : 0.0;
   }
 }
 
@@ -140,14 +141,14 @@ hipError_t launch_setup_inverses(Ctx& c) {
   a.t4 = c.t4;
   a.al = c.alpha;
   a.be = c.beta;
-  a.invV = c.invV;
-  a.invW = c.invW;
+  a.inv = c.inv;
   a.Mx = d.Mx;
   a.N = d.N;
   a.Ny = d.Ny;
   a.ne = d.ne;
   a.no = d.no;
   a.stride = d.inv_stride;
+  a.row = d.inv_row;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
   size_t shm = sizeof(double) * (size_t)d.ne * d.ne;
@@ -163,13 +164,13 @@ hipError_t launch_setup_inverses(Ctx& c) {
 
 struct SolveArgs {
   const double2 *rvE, *rvO, *roE, *roO;  // parity-folded right-hand sides
-  const double *invV, *invW;
+  const double* inv;
   double2 *v, *oy;
   double2 *au0, *w0;
   const double2 *c1n, *c1p, *c3n, *c3p;  // mean Lamb point values, n and n-1
   const double *t0, *t2;
   int Mx, Mz, N, Ny, ne, no;
-  long stride, ncol;
+  long stride, row, ncol;
   double dt, re;
 };
 
@@ -269,8 +270,9 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
     const int rr = t - eq * (pe + po);
     const bool odd = rr >= pe;
     const int i2 = 2 * (odd ? rr - pe : rr);
-    const int n = odd ? no : ne, nP = odd ? noP : neP;
-    const double* M = (eq ? a.invW : a.invV) + (long)g * a.stride + (odd ? (long)ne * neP : 0) + i2;
+    const int n = odd ? no : ne;
+    const long nP = a.row;
+    const double* M = a.inv + (long)g * a.stride + eq * (neP + noP) + (odd ? neP : 0) + i2;
     const double* rhs = rb + eq * 4 * Ny + (odd ? ne : 0);
     double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int j = 0;
@@ -435,8 +437,7 @@ hipError_t launch_solve(Ctx& c) {
   a.rvO = c.rvO;
   a.roE = c.roE;
   a.roO = c.roO;
-  a.invV = c.invV;
-  a.invW = c.invW;
+  a.inv = c.inv;
   a.v = c.v;
   a.oy = c.oy;
   a.au0 = c.au0;
@@ -454,6 +455,7 @@ hipError_t launch_solve(Ctx& c) {
   a.ne = d.ne;
   a.no = d.no;
   a.stride = d.inv_stride;
+  a.row = d.inv_row;
   a.ncol = d.ncol;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
