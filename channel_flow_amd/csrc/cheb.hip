@@ -216,7 +216,11 @@ struct PrepArgs {
   double c;
 };
 
+// The derivative chains split by degree parity: chain p reads v and omega_y at
+// degrees of parity p only (D^1, D^3 land on parity 1-p, D^2, D^4 back on p),
+// so one thread runs one chain of one column, loads issued 8 degrees ahead.
 __global__ __launch_bounds__(256) void k_rhs_prep(PrepArgs a) {
+  const int p = blockIdx.y;
   const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cm >= a.ncol) return;
   const int kz = (int)(cm / a.Mx);
@@ -224,40 +228,50 @@ __global__ __launch_bounds__(256) void k_rhs_prep(PrepArgs a) {
   const double s = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
   const double c = a.c, c2 = 1.0 - 2.0 * c * s, c0 = s * (1.0 - c * s), cw = 1.0 - c * s;
   const int N = a.N;
-  // derivative levels 1..4 of v and 1..2 of omega_y at k+1 and k+2 (unhalved)
-  double2 v1[2] = {}, v2[2] = {}, v3[2] = {}, v4[2] = {}, w1[2] = {}, w2[2] = {};
-  double2 vn = make_double2(0, 0), wn = make_double2(0, 0);  // a_{k+1}
+  // unhalved derivative levels two degrees up (same parity) and the inputs one degree up
+  double2 d1 = {}, d2 = {}, d3 = {}, d4 = {}, e1 = {}, e2 = {};
+  double2 vn = make_double2(0, 0), wn = make_double2(0, 0);  // v_{k+1}, w_{k+1} (parity p)
   auto step = [](double2 up2, double2 lowp1, int k) {          // b_k = b_{k+2} + 2(k+1) a_{k+1}
     const double f = 2.0 * (k + 1);
     return make_double2(up2.x + f * lowp1.x, up2.y + f * lowp1.y);
   };
-  for (int k = N; k >= 0; --k) {
-    const long off = (long)k * a.ncol + cm;
-    const double2 vk = a.v[off], wk = a.oy[off];
-    double2 d1, d2, d3, d4, e1, e2;
-    if (k == N) {
-      d1 = d2 = d3 = d4 = e1 = e2 = make_double2(0, 0);
-    } else {
-      d1 = step(v1[1], vn, k);
-      d2 = step(v2[1], v1[0], k);
-      d3 = step(v3[1], v2[0], k);
-      d4 = step(v4[1], v3[0], k);
-      e1 = step(w1[1], wn, k);
-      e2 = step(w2[1], w1[0], k);
+  for (int k0 = N; k0 >= 0; k0 -= 8) {
+    double2 vin[8], win[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 - t;
+      if (k >= 0 && (k & 1) == p) {
+        const long off = (long)k * a.ncol + cm;
+        vin[t] = a.v[off];
+        win[t] = a.oy[off];
+      }
     }
-    v1[1] = v1[0]; v1[0] = d1;
-    v2[1] = v2[0]; v2[0] = d2;
-    v3[1] = v3[0]; v3[0] = d3;
-    v4[1] = v4[0]; v4[0] = d4;
-    w1[1] = w1[0]; w1[0] = e1;
-    w2[1] = w2[0]; w2[0] = e2;
-    if (k == 0) {  // c_0 = 2
-      d2.x *= 0.5; d2.y *= 0.5; d4.x *= 0.5; d4.y *= 0.5; e2.x *= 0.5; e2.y *= 0.5;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 - t;
+      if (k < 0) break;
+      if ((k & 1) != p) {
+        // odd levels at degree k from the even levels (and inputs) at k + 1
+        d1 = step(d1, vn, k);
+        d3 = step(d3, d2, k);
+        e1 = step(e1, wn, k);
+      } else {
+        const double2 vk = vin[t], wk = win[t];
+        double2 q2 = step(d2, d1, k), q4 = step(d4, d3, k), r2 = step(e2, e1, k);
+        if (k == N) q2 = q4 = r2 = make_double2(0, 0);
+        d2 = q2;
+        d4 = q4;
+        e2 = r2;
+        if (k == 0) {  // c_0 = 2
+          q2.x *= 0.5; q2.y *= 0.5; q4.x *= 0.5; q4.y *= 0.5; r2.x *= 0.5; r2.y *= 0.5;
+        }
+        const long off = (long)k * a.ncol + cm;
+        a.gv[off] = make_double2(c * q4.x + c2 * q2.x - c0 * vk.x, c * q4.y + c2 * q2.y - c0 * vk.y);
+        a.go[off] = make_double2(c * r2.x + cw * wk.x, c * r2.y + cw * wk.y);
+        vn = vk;
+        wn = wk;
+      }
     }
-    a.gv[off] = make_double2(c * d4.x + c2 * d2.x - c0 * vk.x, c * d4.y + c2 * d2.y - c0 * vk.y);
-    a.go[off] = make_double2(c * e2.x + cw * wk.x, c * e2.y + cw * wk.y);
-    vn = vk;
-    wn = wk;
   }
 }
 
@@ -308,7 +322,7 @@ hipError_t launch_rhs_prep(Ctx& c) {
   a.N = c.d.N;
   a.ncol = c.d.ncol;
   a.c = c.prm.dt * 0.5 / c.prm.re;
-  hipLaunchKernelGGL(k_rhs_prep, dim3((unsigned)((a.ncol + 255) / 256)), dim3(256), 0, c.stream, a);
+  hipLaunchKernelGGL(k_rhs_prep, dim3((unsigned)((a.ncol + 255) / 256), 2), dim3(256), 0, c.stream, a);
   return hipGetLastError();
 }
 
@@ -361,9 +375,25 @@ __global__ __launch_bounds__(256) void k_hassemble(HArgs a) {
   double2 bx2 = make_double2(0, 0), bx1 = make_double2(0, 0);
   double2 bz2 = make_double2(0, 0), bz1 = make_double2(0, 0);
   double2 lxn = make_double2(0, 0), lzn = make_double2(0, 0);  // a_{k+1}
-  for (int k = N; k >= 0; --k) {
+  for (int k0 = N; k0 >= 0; k0 -= 4) {
+    // loads issued four degrees ahead of the recurrence
+    double2 pl[4][3];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int k = k0 - t;
+      if (k >= 0) {
+        const long off = (long)k * a.ncol + cm;
+        pl[t][0] = a.lx[off];
+        pl[t][1] = a.ly[off];
+        pl[t][2] = a.lz[off];
+      }
+    }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int k = k0 - t;
+    if (k < 0) break;
     const long off = (long)k * a.ncol + cm;
-    const double2 lx = a.lx[off], ly = a.ly[off], lz = a.lz[off];
+    const double2 lx = pl[t][0], ly = pl[t][1], lz = pl[t][2];
     double2 dx, dz;
     if (k == N) {
       dx = make_double2(0, 0);
@@ -395,6 +425,7 @@ __global__ __launch_bounds__(256) void k_hassemble(HArgs a) {
     a.hg[off] = make_double2(hgr, hgi);
     lxn = lx;
     lzn = lz;
+  }
   }
 }
 
