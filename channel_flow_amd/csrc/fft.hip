@@ -588,7 +588,10 @@ __global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(RegZ<V> a) {
   const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
   V* buf = sm + L + tx * Gm::BUF_A;
   const int Mz = a.Mz, h = Mz / 2;
-  const int kx = blockIdx.x * TX + tx;
+  // kx groups in XCD-contiguous order: the groups that share 128-byte lines of
+  // narrow (FP32) rows go through the same L2
+  const int kg = (gridDim.x & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+  const int kx = kg * TX + tx;
   const int p = blockIdx.y;
   const V* in = a.in[blockIdx.z] + (long)p * a.in_plane;
   V* out = a.out[blockIdx.z] + (long)p * a.out_plane;
