@@ -133,6 +133,52 @@ def copy_probe(nbytes=1 << 30, reps=5):
     return 0
 
 
+def run_ensemble(steps, warmup):
+    """The batched-ensemble workload: 24 channels of mixed sizes and Reynolds
+    numbers, one hipGraph launch per step, against the same members stepped by
+    eager launches on their own streams.  Prints its own JSON line."""
+    from channel_flow_amd import ic
+    from channel_flow_amd.ensemble import Ensemble
+    members = ic.ensemble_members()
+    with Ensemble(members) as ens:
+        for i, m in enumerate(members):
+            ens.member(i).initial(*ic.seeded_ic(m["nnx"], m["nny"], m["nnz"], m["aphi"], m["beta"], m["u_c"],
+                                                m["rms"], m["kmax"], m["slope"], m["seed"]))
+        ens.step(warmup)
+        t0 = time.perf_counter()
+        ens.step(steps)
+        t_graph = time.perf_counter() - t0
+        # same members, eager launches: each member's steps queued on its own stream
+        for i in range(len(members)):
+            ens.member(i).step(warmup)
+        for i in range(len(members)):
+            ens.member(i).sync()
+        t0 = time.perf_counter()
+        for i in range(len(members)):
+            ens.member(i).step(steps)
+        for i in range(len(members)):
+            ens.member(i).sync()
+        t_eager = time.perf_counter() - t0
+        finite = all(np.all(np.isfinite(ens.member(i).get_state()["vnp1"])) for i in range(len(members)))
+    sizes = {}
+    for m in members:
+        key = f"{m['nnx']}x{m['nnz'] + 1}x{m['nny']}"
+        sizes[key] = sizes.get(key, 0) + 1
+    out = {
+        "metric": "ensemble timesteps/s (24 channels advanced together, one hipGraph launch per step)",
+        "value": round(steps / t_graph, 3), "unit": "ensemble steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(t_graph * 1e3 / steps, 4), "higher_is_better": True,
+        "member_steps_per_s": round(len(members) * steps / t_graph, 1),
+        "eager_streams_ms_per_step": round(t_eager * 1e3 / steps, 4),
+        "dtype": "f32 transforms and products, f64 solves and state", "data": "synthetic",
+        "config": {"workload": "ensemble24: re_c in [2000, 6000], dt=1e-3, mixed precision",
+                   "members": sizes},
+        "state_finite": bool(finite),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def cpu_baseline(cfg, seconds_budget=30.0):
     """Oracle (numpy restatement of the reference) on one core, bounded sample.
 
@@ -257,12 +303,16 @@ def main(argv=None):
     ap.add_argument("--error-steps", type=int, default=100,
                     help="steps after which the mixed form is compared with the FP64 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="run the 24-channel batched-ensemble workload instead (its own JSON line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration)")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     if a.copy_probe:
         return copy_probe()
+    if a.ensemble and a.gpus == 1:
+        return run_ensemble(a.steps, a.warmup)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the whole channel lives on one device "
               "(multi-GPU sharding is a design note, not a run)", file=sys.stderr)

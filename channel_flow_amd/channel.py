@@ -29,10 +29,25 @@ class Channel:
         if rc != 0:
             msg = self._lib.chf_last_error(None).decode()
             raise rt.ChfError(f"chf_create failed ({rc}): {msg}")
+        self._owned = True
+        self._init_dims(precision)
+
+    def _init_dims(self, precision):
         mx, mz, ny = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._lib.chf_dims(self._ctx, ctypes.byref(mx), ctypes.byref(mz), ctypes.byref(ny))
         self.Mx, self.Mz, self.Ny = mx.value, mz.value, ny.value
         self.precision = precision
+
+    @classmethod
+    def _borrowed(cls, lib, ctx, params, precision):
+        """A view of a context owned elsewhere (an ensemble member)."""
+        self = cls.__new__(cls)
+        self._lib = lib
+        self._ctx = ctx
+        self.params = params
+        self._owned = False
+        self._init_dims(precision)
+        return self
 
     # -------------------------------------------------------------- plumbing
     def _check(self, rc, what):
@@ -41,9 +56,9 @@ class Channel:
             raise rt.ChfError(f"{what} failed ({rc}): {msg}")
 
     def close(self):
-        if self._ctx:
+        if self._ctx and self._owned:
             self._lib.chf_destroy(self._ctx)
-            self._ctx = ctypes.c_void_p()
+        self._ctx = ctypes.c_void_p()
 
     def __del__(self):
         try:

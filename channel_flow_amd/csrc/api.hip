@@ -573,3 +573,196 @@ int chf_timing(void* ctx, int id, double* total_ms, int* launches) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ ensemble
+//
+// Independent channels of any mix of sizes, Reynolds numbers and precisions
+// advanced together.  One step of all members is one hipGraph launch: the
+// members' kernel sequences [solve; nonlinear] are captured as parallel
+// branches (each member on its own stream, forked from and joined to the
+// ensemble stream), so small members fill the GPU together.  The AB2 history
+// buffers swap every step, so a graph is captured per combination of the
+// members' history phases (normally two) and cached.
+
+namespace {
+
+struct Ens {
+  std::vector<Ctx*> m;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  std::vector<std::pair<std::vector<int>, hipGraphExec_t>> graphs;
+  std::string err;
+};
+
+int ens_fail(Ens* e, const std::string& msg, int rc = -1) {
+  e->err = msg;
+  return rc;
+}
+
+#define ENS_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return ens_fail(e, std::string(#expr) + ": " + hipGetErrorString(e_), (int)e_); \
+  } while (0)
+
+void ens_free(Ens* e) {
+  for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.second);
+  e->graphs.clear();
+  for (Ctx* c : e->m) {
+    if (!c) continue;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+  }
+  e->m.clear();
+  for (hipEvent_t ev : e->join) (void)hipEventDestroy(ev);
+  e->join.clear();
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->origin) (void)hipStreamDestroy(e->origin);
+}
+
+// Captures one step of every member; member host state (history phase) is
+// restored afterwards, the caller advances it per launch.
+int ens_capture(Ens* e, hipGraphExec_t* out) {
+  std::vector<int> phase;
+  for (Ctx* c : e->m) phase.push_back(c->hcur);
+  ENS_TRY(hipStreamBeginCapture(e->origin, hipStreamCaptureModeRelaxed));
+  int rc = 0;
+  std::string what;
+  hipError_t he = hipEventRecord(e->fork, e->origin);
+  for (size_t i = 0; i < e->m.size() && he == hipSuccess && rc == 0; ++i) {
+    Ctx* c = e->m[i];
+    he = hipStreamWaitEvent(c->stream, e->fork, 0);
+    if (he != hipSuccess) break;
+    rc = solve(c);
+    if (rc == 0) rc = nonlinear(c);
+    if (rc) {
+      what = c->err;
+      break;
+    }
+    he = hipEventRecord(e->join[i], c->stream);
+    if (he == hipSuccess) he = hipStreamWaitEvent(e->origin, e->join[i], 0);
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(e->origin, &graph);
+  for (size_t i = 0; i < e->m.size(); ++i) e->m[i]->hcur = phase[i];
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return ens_fail(e, "capture of a member step failed: " + what, rc);
+  }
+  if (he != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return ens_fail(e, std::string("capture: ") + hipGetErrorString(he), (int)he);
+  }
+  ENS_TRY(ee);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  ENS_TRY(ie);
+  *out = exec;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chf_ensemble_create(const chf_params* members, int n, void** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  if (!members || n <= 0) {
+    g_create_err = "chf_ensemble_create: no members";
+    return -1;
+  }
+  for (int i = 0; i < n; ++i) {
+    std::string msg;
+    if (validate(&members[i], msg)) {
+      g_create_err = "member " + std::to_string(i) + ": " + msg;
+      return -1;
+    }
+  }
+  Ens* e = new Ens();
+  for (int i = 0; i < n; ++i) {
+    Ctx* c = new Ctx();
+    c->prm = members[i];
+    const int rc = build(c);
+    if (rc) {
+      g_create_err = "member " + std::to_string(i) + ": " + c->err;
+      free_all(c);
+      delete c;
+      ens_free(e);
+      delete e;
+      return rc;
+    }
+    e->m.push_back(c);
+  }
+  hipError_t he = hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+  e->join.assign(n, nullptr);
+  for (int i = 0; i < n && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming);
+  if (he != hipSuccess) {
+    g_create_err = std::string("ensemble streams/events: ") + hipGetErrorString(he);
+    ens_free(e);
+    delete e;
+    return (int)he;
+  }
+  *out = e;
+  return 0;
+}
+
+int chf_ensemble_destroy(void* ens) {
+  if (!ens) return -1;
+  Ens* e = static_cast<Ens*>(ens);
+  if (e->origin) (void)hipStreamSynchronize(e->origin);
+  ens_free(e);
+  delete e;
+  return 0;
+}
+
+const char* chf_ensemble_last_error(void* ens) {
+  if (!ens) return g_create_err.c_str();
+  return static_cast<Ens*>(ens)->err.c_str();
+}
+
+int chf_ensemble_size(void* ens) {
+  if (!ens) return -1;
+  return (int)static_cast<Ens*>(ens)->m.size();
+}
+
+int chf_ensemble_member(void* ens, int i, void** ctx) {
+  if (!ens || !ctx) return -1;
+  Ens* e = static_cast<Ens*>(ens);
+  if (i < 0 || i >= (int)e->m.size()) return ens_fail(e, "chf_ensemble_member: index out of range");
+  *ctx = e->m[i];
+  return 0;
+}
+
+int chf_ensemble_step(void* ens, int n) {
+  if (!ens) return -1;
+  Ens* e = static_cast<Ens*>(ens);
+  if (n < 0) return ens_fail(e, "chf_ensemble_step: negative step count");
+  for (Ctx* c : e->m) {
+    if (c->timing) return ens_fail(e, "chf_ensemble_step: turn member timing off first");
+    // member work queued outside the graph (set_state, init) must finish first
+    ENS_TRY(hipStreamSynchronize(c->stream));
+  }
+  for (int s = 0; s < n; ++s) {
+    std::vector<int> phase;
+    for (Ctx* c : e->m) phase.push_back(c->hcur);
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : e->graphs)
+      if (g.first == phase) exec = g.second;
+    if (!exec) {
+      const int rc = ens_capture(e, &exec);
+      if (rc) return rc;
+      e->graphs.emplace_back(phase, exec);
+    }
+    ENS_TRY(hipGraphLaunch(exec, e->origin));
+    for (Ctx* c : e->m) c->hcur = 1 - c->hcur;  // what nonlinear() does per step
+  }
+  ENS_TRY(hipStreamSynchronize(e->origin));
+  return 0;
+}
+
+}  // extern "C"

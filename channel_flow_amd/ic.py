@@ -133,3 +133,23 @@ CONFIGS = {
                              re_p=2.0 / 12500.0, dt=2.5e-4, u_c=1.0, rms=0.1, kmax=16.0,
                              slope=-4.0 / 3.0, seed=1005, steps=50),
 }
+
+
+# Batched ensemble workload: 24 independent channels, resolutions drawn
+# uniformly from {64x65x64, 96x97x96, 128x97x128} (nnx x (N+1) x nny), centreline
+# Reynolds number uniform in [2000, 6000] with re_p = 2/re (the parabola is the
+# steady state), dt = 1e-3, mixed precision, 500 steps; member m uses IC seed
+# 2000 + m.
+ENSEMBLE_SIZES = ((64, 64, 64), (96, 96, 96), (128, 128, 96))  # (nnx, nny, nnz)
+
+
+def ensemble_members(n=24, seed=4000, precision="mixed"):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for m in range(n):
+        nnx, nny, nnz = ENSEMBLE_SIZES[int(rng.integers(0, len(ENSEMBLE_SIZES)))]
+        re = float(rng.uniform(2000.0, 6000.0))
+        out.append(dict(nnx=nnx, nny=nny, nnz=nnz, aphi=1.0, beta=0.5, re=re, re_p=2.0 / re, dt=1.0e-3,
+                        u_c=1.0, rms=0.1, kmax=16.0, slope=-4.0 / 3.0, seed=2000 + m,
+                        precision=precision))
+    return out

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_PKG, "lib", "libchf.so")
 EXPORTS = ("chf_create", "chf_destroy", "chf_last_error", "chf_set_state", "chf_get_state",
            "chf_init_fields", "chf_nonlinear", "chf_solve", "chf_step", "chf_sync",
            "chf_get_fields", "chf_dims", "chf_device_bytes", "chf_set_timing", "chf_kernel_count",
-           "chf_kernel_name", "chf_timing")
+           "chf_kernel_name", "chf_timing", "chf_ensemble_create", "chf_ensemble_destroy",
+           "chf_ensemble_last_error", "chf_ensemble_size", "chf_ensemble_member", "chf_ensemble_step")
 
 
 class ChfParams(ctypes.Structure):
@@ -70,6 +71,13 @@ def load():
     lib.chf_kernel_name.restype = ctypes.c_char_p
     lib.chf_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_int)]
+    lib.chf_ensemble_create.argtypes = [ctypes.POINTER(ChfParams), ctypes.c_int, ctypes.POINTER(vp)]
+    lib.chf_ensemble_destroy.argtypes = [vp]
+    lib.chf_ensemble_last_error.argtypes = [vp]
+    lib.chf_ensemble_last_error.restype = ctypes.c_char_p
+    lib.chf_ensemble_size.argtypes = [vp]
+    lib.chf_ensemble_member.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.chf_ensemble_step.argtypes = [vp, ctypes.c_int]
     _lib = lib
     return lib
 

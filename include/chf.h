@@ -99,6 +99,22 @@ int chf_kernel_count(void);
 const char* chf_kernel_name(int id);
 int chf_timing(void* ctx, int id, double* total_ms, int* launches);
 
+/* Ensemble: n independent channels (any mix of grid sizes, Reynolds numbers
+ * and precisions) advanced together, one step of every member being one
+ * hipGraph launch in which the members run as parallel branches.  The
+ * reference runs one channel per job, so these have no single counterpart;
+ * each member is a full context whose state is exchanged with the calls
+ * above (chf_set_state, chf_init_fields, chf_get_state, chf_get_fields), and
+ * one ensemble step is chf_step(member, 1) for every member (PF:220-532).
+ * chf_ensemble_step returns after the steps have completed.  A member context
+ * belongs to its ensemble: do not pass it to chf_destroy. */
+int chf_ensemble_create(const chf_params* members, int n, void** ens);
+int chf_ensemble_destroy(void* ens);
+const char* chf_ensemble_last_error(void* ens);
+int chf_ensemble_size(void* ens);
+int chf_ensemble_member(void* ens, int i, void** ctx);
+int chf_ensemble_step(void* ens, int nsteps);
+
 #ifdef __cplusplus
 }
 #endif
