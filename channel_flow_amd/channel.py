@@ -151,6 +151,30 @@ class Channel:
                     "chf_get_fields")
         return out
 
+    FIELDS = ("u", "v", "w", "omega_x", "omega_y", "omega_z")
+
+    def physical(self, field) -> np.ndarray:
+        """transform(1, .) (PF:948-961): the unpadded physical field after the
+        last solve, double[Ny][nny][nnx] at y_k = cos(pi k/N)."""
+        i = self.FIELDS.index(field) if isinstance(field, str) else int(field)
+        out = np.empty((self.Ny, self.Mz, 2 * self.Mx), dtype=np.float64)
+        self._check(self._lib.chf_physical(self._ctx, i, self._dp(out)), "chf_physical")
+        return out
+
+    def plane_statistics(self) -> np.ndarray:
+        """The reductions of statistics (PF:2816-3020): double[12][Ny], rows
+        ua va wa uu vv ww uv uw vw max|u| max|v| max|w|."""
+        out = np.empty((12, self.Ny), dtype=np.float64)
+        self._check(self._lib.chf_statistics(self._ctx, self._dp(out)), "chf_statistics")
+        return out
+
+    def statistics(self) -> dict:
+        """The quantities statistics prints and writes (PF:2816-3190)."""
+        from . import statistics
+        p = self.params
+        return statistics.diagnostics(self.plane_statistics(), p.nnx, p.nny, p.nnz, p.aphi, p.beta, p.dt,
+                                      p.re, p.re_p)
+
     def device_bytes(self) -> int:
         return int(self._lib.chf_device_bytes(self._ctx))
 

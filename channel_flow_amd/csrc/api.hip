@@ -148,7 +148,8 @@ void free_all(Ctx* c) {
   f(c->hvc); f(c->hgc);
   f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
-  f(c->inv); f(c->twx); f(c->twz); f(c->costab);
+  f(c->inv);
+  f(c->phys_c); f(c->phys_r[0]); f(c->phys_r[1]); f(c->phys_r[2]); f(c->stats); f(c->twxu); f(c->twzu); f(c->twx); f(c->twz); f(c->costab);
   f(c->gv); f(c->go);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
@@ -543,6 +544,78 @@ int chf_set_timing(void* ctx, int on) {
   c->timing = on != 0;
   c->marks.clear();
   c->pool_next = 0;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Buffers of the unpadded physical transform, allocated on first use (they
+// are not part of the timestep).
+int ensure_physical(Ctx* c) {
+  if (c->stats) return 0;
+  const Dims& d = c->d;
+  const size_t plane = (size_t)d.Ny * d.Mz * 2 * d.Mx;
+  CHF_TRY(dalloc(c, &c->phys_c, (size_t)d.Ny * d.ncol));
+  for (auto& p : c->phys_r) CHF_TRY(dalloc(c, &p, plane));
+  auto tw = [](int L) {
+    std::vector<double2> t(L);
+    for (int m = 0; m < L; ++m)
+      t[m] = make_double2(cospi_frac(2L * m, L), -std::sin(2.0 * M_PI * (double)m / (double)L));
+    return t;
+  };
+  const auto tx = tw(2 * d.Mx), tz = tw(d.Mz);
+  CHF_TRY(dalloc(c, &c->twxu, tx.size()));
+  CHF_TRY(dalloc(c, &c->twzu, tz.size()));
+  CHF_TRY(hipMemcpy(c->twxu, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice));
+  CHF_TRY(hipMemcpy(c->twzu, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice));
+  CHF_TRY(dalloc(c, &c->stats, 12 * (size_t)d.Ny));
+  return 0;
+}
+
+const double2* field_ptr(Ctx* c, int field) {
+  switch (field) {
+    case 0: return c->u;
+    case 1: return c->v;
+    case 2: return c->w;
+    case 3: return c->ox;
+    case 4: return c->oy;
+    case 5: return c->oz;
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int chf_physical(void* ctx, int field, double* out) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  const double2* f = field_ptr(c, field);
+  if (!f) return fail(c, "chf_physical: field must be 0..5 (u, v, w, omega_x, omega_y, omega_z)");
+  if (!out) return fail(c, "chf_physical: null output");
+  const int rc = ensure_physical(c);
+  if (rc) return rc;
+  CHF_TRY(launch_physical(*c, f, c->phys_r[0]));
+  const size_t n = (size_t)c->d.Ny * c->d.Mz * 2 * c->d.Mx;
+  CHF_TRY(hipMemcpyAsync(out, c->phys_r[0], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int chf_statistics(void* ctx, double* out) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!out) return fail(c, "chf_statistics: null output");
+  const int rc = ensure_physical(c);
+  if (rc) return rc;
+  for (int q = 0; q < 3; ++q) CHF_TRY(launch_physical(*c, field_ptr(c, q), c->phys_r[q]));
+  CHF_TRY(launch_plane_stats(*c, c->phys_r[0], c->phys_r[1], c->phys_r[2], c->stats));
+  CHF_TRY(hipMemcpyAsync(out, c->stats, 12 * (size_t)c->d.Ny * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  CHF_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
 

@@ -103,6 +103,11 @@ struct Ctx {
   double* inv = nullptr;
   double2* twx = nullptr;  // exp(-2 pi i m/Lx), m < Lx
   double2* twz = nullptr;  // exp(-2 pi i m/Lz), m < Lz
+  // unpadded physical output and plane statistics (allocated on first use)
+  double2* phys_c = nullptr;                          // [Ny][Mz][Mx] work
+  double* phys_r[3] = {nullptr, nullptr, nullptr};    // [Ny][Mz][2 Mx] physical u, v, w
+  double* stats = nullptr;                            // [12][Ny]
+  double2 *twxu = nullptr, *twzu = nullptr;           // unpadded twiddles (2 Mx, Mz)
   long long bytes = 0;
   std::string err;
   // optional per-launch HIP-event timing on the context stream
@@ -147,6 +152,10 @@ hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const*
 hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_hassemble(Ctx& c);
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf);
+hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);
+hipError_t launch_stream_c2r_unpadded(Ctx& c, const double2* in, double* out);
+hipError_t launch_physical(Ctx& c, const double2* field, double* out);
+hipError_t launch_plane_stats(Ctx& c, const double* u, const double* v, const double* w, double* out);
 hipError_t launch_mean_points(Ctx& c);
 hipError_t launch_rhs_prep(Ctx& c);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y

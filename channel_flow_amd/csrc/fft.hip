@@ -213,6 +213,89 @@ static hipError_t launch_z(Ctx& c, double2* const* in, double2* const* out, int 
   return hipGetLastError();
 }
 
+// ------------------------------------------------ unpadded physical output
+
+// Streamwise c2r of LPB (y point, span point) lines per block:
+// in [Ny][Mz][Mx] complex -> out [Ny][Mz][nnx] real.
+struct XPArgs {
+  const double2* in;
+  double* out;
+  int Mx, L, LPB;
+  long nlines;
+  FftPlan pl;
+  const double2* tw;
+};
+
+__global__ __launch_bounds__(256) void k_xphys(XPArgs a) {
+  extern __shared__ double2 sm[];
+  const int L = a.L, Mx = a.Mx, LPB = a.LPB;
+  double2* b0 = sm;
+  double2* b1 = sm + LPB * L;
+  const long line0 = (long)blockIdx.x * LPB;
+  for (int idx = threadIdx.x; idx < LPB * L; idx += blockDim.x) {
+    const int ln = idx / L, m = idx - ln * L;
+    const long line = line0 + ln;
+    double2 val = make_double2(0.0, 0.0);
+    if (line < a.nlines) {
+      const double2* f = a.in + line * Mx;
+      if (m == 0) {
+        val = make_double2(f[0].x, 0.0);            // Re of the kx = 0 entry
+      } else if (m < Mx) {
+        val = f[m];
+      } else if (m > L - Mx) {
+        const double2 g = f[L - m];                 // getall: F(-m) = conj F(m)
+        val = make_double2(g.x, -g.y);
+      }                                             // m = Mx: Nyquist, zero
+    }
+    b0[idx] = val;
+  }
+  __syncthreads();
+  double2* r = fft_lds(b0, b1, a.pl, LPB, +1, a.tw);
+  for (int idx = threadIdx.x; idx < LPB * L; idx += blockDim.x) {
+    const int ln = idx / L, m = idx - ln * L;
+    const long line = line0 + ln;
+    if (line < a.nlines) a.out[line * L + m] = r[idx].x;
+  }
+}
+
+
+hipError_t launch_stream_c2r_unpadded(Ctx& c, const double2* in, double* out) {
+  const Dims& d = c.d;
+  XPArgs a{};
+  a.in = in;
+  a.out = out;
+  a.Mx = d.Mx;
+  a.L = 2 * d.Mx;
+  a.LPB = a.L >= 2048 ? 1 : 2048 / a.L;
+  a.nlines = (long)d.Ny * d.Mz;
+  a.pl = make_plan(a.L);
+  a.tw = c.twxu;
+  const size_t shm = 2ull * a.LPB * a.L * sizeof(double2);
+  hipLaunchKernelGGL(k_xphys, dim3((unsigned)((a.nlines + a.LPB - 1) / a.LPB)), dim3(256), shm, c.stream, a);
+  return hipGetLastError();
+}
+
+// transform(1, .) span step (fourtrx_zcomp, PF:790-803): Mz span modes ->
+// Mz span points per (wall-normal point, kx), Nyquist slot zeroed, in place.
+hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout) {
+  const Dims& d = c.d;
+  ZArgs a{};
+  a.in[0] = inout;
+  a.out[0] = inout;
+  a.Mx = d.Mx;
+  a.Mz = d.Mz;
+  a.Lz = d.Mz;
+  a.TX = pick_tx(d.Mx, d.Mz);
+  a.in_plane = (long)d.Mz * d.Mx;
+  a.out_plane = a.in_plane;
+  a.pl = make_plan(d.Mz);
+  a.tw = c.twzu;
+  dim3 grid(d.Mx / a.TX, d.Ny, 1);
+  const size_t shm = 2ull * a.TX * d.Mz * sizeof(double2);
+  hipLaunchKernelGGL(k_zfft_inv, grid, dim3(256), shm, c.stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_zfft_inv(Ctx& c, double2* const* in, double2* const* out, int nf) {
   return launch_z(c, in, out, nf, true);
 }

@@ -83,6 +83,19 @@ int chf_sync(void* ctx);
 int chf_get_fields(void* ctx, double* unp1, double* wnp1, double* omegaxnp1,
                    double* omegaznp1);
 
+/* Unpadded physical field (replaces transform(1, .), PF:948-961 with
+ * fourtrx_zcomp(+1), PF:774-860, as statistics calls it at PF:2723-2725):
+ * field 0..5 = u, v, w, omega_x, omega_y, omega_z after the last solve, out =
+ * real*8 (0:nnx-1, 0:nny-1, 0:N) == double[Ny][nny][nnx] at x_i = 2 pi aphi i/nnx,
+ * z_j = 2 pi beta j/nny, y_k = cos(pi k/N). */
+int chf_physical(void* ctx, int field, double* out);
+
+/* Plane statistics of the physical u, v, w after the last solve (the
+ * reductions of statistics, PF:2816-3020): out = double[12][Ny], rows
+ * ua va wa uu vv ww uv uw vw max|u| max|v| max|w| per y_k (means, covariances
+ * about the plane mean, and extrema over each plane). */
+int chf_statistics(void* ctx, double* out);
+
 /* Sizes derived from the parameters (for callers that allocate buffers). */
 int chf_dims(void* ctx, int* mx, int* mz, int* ny);
 

@@ -577,3 +577,43 @@ def initial_presolve_state(v: np.ndarray, oy: np.ndarray, au0: np.ndarray, w0: n
     kinematics_all(st, p)
     getnonlinear(st, p)
     return st
+
+
+# --------------------------------------------------------------------------
+# Unpadded physical output and plane statistics (test infrastructure)
+
+
+def physical_field(F, p: Params):
+    """transform(1, .) (PF:948-961, fourtrx_zcomp(+1) PF:774-860) of one
+    spectral field F = complex[Ny][Mz][Mx] (coefficients): values at
+    y_k = cos(pi k/N) (plain Chebyshev sum, PF:951-957), span inverse with the
+    Nyquist slot zeroed (PF:792-794), streamwise Hermitian fill and Re
+    (getall PF:1350-1362, PF:806-818).  Returns real[Ny][nny][nnx]."""
+    F = np.asarray(F, dtype=np.complex128)
+    Ny, Mz, Mx = F.shape
+    N = Ny - 1
+    k = np.arange(Ny)
+    T = np.cos(np.pi * np.outer(k, k) / N)
+    pts = np.einsum("kl,lzx->kzx", T, F)
+    pts[:, Mz // 2, :] = 0.0
+    S = np.fft.ifft(pts, axis=1) * Mz                 # e^{+i}, unnormalised
+    nnx = 2 * Mx
+    G = np.zeros((Ny, Mz, nnx), dtype=np.complex128)
+    G[:, :, 0] = S[:, :, 0].real
+    G[:, :, 1:Mx] = S[:, :, 1:Mx]
+    G[:, :, nnx - Mx + 1:] = np.conj(S[:, :, 1:Mx][:, :, ::-1])
+    return (np.fft.ifft(G, axis=2) * nnx).real
+
+
+def plane_statistics(u, v, w):
+    """Plane means, covariances about the plane mean and extrema of physical
+    u, v, w (statistics, PF:2825-2868 and PF:2869-3020): real[12][Ny], rows
+    ua va wa uu vv ww uv uw vw max|u| max|v| max|w|."""
+    out = []
+    m = [f.mean(axis=(1, 2)) for f in (u, v, w)]
+    d = [f - mm[:, None, None] for f, mm in zip((u, v, w), m)]
+    out += m
+    out += [(d[0] * d[0]).mean(axis=(1, 2)), (d[1] * d[1]).mean(axis=(1, 2)), (d[2] * d[2]).mean(axis=(1, 2)),
+            (d[0] * d[1]).mean(axis=(1, 2)), (d[0] * d[2]).mean(axis=(1, 2)), (d[1] * d[2]).mean(axis=(1, 2))]
+    out += [np.abs(f).max(axis=(1, 2)) for f in (u, v, w)]
+    return np.array(out)

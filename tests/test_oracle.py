@@ -352,3 +352,56 @@ def test_tollmien_schlichting_growth_through_full_step():
     c_num = 1j * np.log(r) / (1.0 * dt)
     assert abs(c_num.real - c_ref.real) < 2e-4
     assert abs(c_num.imag - c_ref.imag) < 2e-4
+
+
+def test_physical_field_is_the_fourier_chebyshev_series():
+    """transform(1, .) restated: brute-force evaluation of the series
+    Re(S(0)) + 2 sum_{kx>0} Re(S(kx) e^{i alpha x}), S = sum over span modes
+    (Nyquist excluded) of sum_l F T_l(y_k) e^{i beta z}, at every grid point."""
+    rng = np.random.default_rng(11)
+    p = small_params(nnx=8, nny=6, nnz=5)
+    Ny, Mz, Mx = p.nnz + 1, p.nny, p.nnx // 2
+    F = rng.standard_normal((Ny, Mz, Mx)) + 1j * rng.standard_normal((Ny, Mz, Mx))
+    got = O.physical_field(F, p)
+    N = p.nnz
+    ref = np.zeros((Ny, Mz, 2 * Mx))
+    for k in range(Ny):
+        y = np.cos(np.pi * k / N)
+        for j in range(Mz):
+            for i in range(2 * Mx):
+                S = np.zeros(Mx, complex)
+                for kz in range(Mz):
+                    if kz == Mz // 2:
+                        continue
+                    kzs = kz if kz < Mz // 2 else kz - Mz
+                    for l in range(Ny):
+                        S += F[l, kz, :] * np.cos(l * np.arccos(y)) * np.exp(2j * np.pi * kzs * j / Mz)
+                val = S[0].real + 2 * sum((S[m] * np.exp(2j * np.pi * m * i / (2 * Mx))).real for m in range(1, Mx))
+                ref[k, j, i] = val
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+def test_plane_statistics_single_modes():
+    """u = U(y) + A(y) cos(x), v = B(y) cos(x), w = C(y) sin(x) on a uniform
+    x grid: means U, 0, 0; uu = A^2/2, vv = B^2/2, ww = C^2/2, uv = AB/2,
+    uw = vw = 0; max|u| = |U| + |A| (x = 0 or pi lies on the grid)."""
+    nnx, nny, Ny = 8, 4, 5
+    y = np.cos(np.pi * np.arange(Ny) / (Ny - 1))
+    x = 2 * np.pi * np.arange(nnx) / nnx
+    U, A, B, C = 1 - y ** 2, 0.3 * y, 0.2 + 0 * y, 0.1 * y ** 2
+    shape = (Ny, nny, nnx)
+    u = np.broadcast_to(U[:, None, None] + A[:, None, None] * np.cos(x), shape)
+    v = np.broadcast_to(B[:, None, None] * np.cos(x), shape)
+    w = np.broadcast_to(C[:, None, None] * np.sin(x), shape)
+    s = O.plane_statistics(u, v, w)
+    tol = dict(rtol=0, atol=1e-14)
+    np.testing.assert_allclose(s[0], U, **tol)
+    np.testing.assert_allclose(s[1], 0, **tol)
+    np.testing.assert_allclose(s[2], 0, **tol)
+    np.testing.assert_allclose(s[3], A ** 2 / 2, **tol)
+    np.testing.assert_allclose(s[4], B ** 2 / 2, **tol)
+    np.testing.assert_allclose(s[5], C ** 2 / 2, **tol)
+    np.testing.assert_allclose(s[6], A * B / 2, **tol)
+    np.testing.assert_allclose(s[7], 0, **tol)
+    np.testing.assert_allclose(s[8], 0, **tol)
+    np.testing.assert_allclose(s[9], np.abs(U) + np.abs(A), **tol)
