@@ -85,6 +85,21 @@ module chf_binding
       type(c_ptr), value :: ctx, unp1, wnp1, omegaxnp1, omegaznp1
     end function chf_get_fields
 
+    ! transform(1, .) of field 0..5 (u, v, w, omega_x, omega_y, omega_z), PF:948-961
+    integer(c_int) function chf_physical(ctx, field, out) bind(C, name="chf_physical")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      real(c_double), intent(out) :: out(*)
+    end function chf_physical
+
+    ! plane statistics of statistics (PF:2816-3020): out(0:nz, 12)
+    integer(c_int) function chf_statistics(ctx, out) bind(C, name="chf_statistics")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(*)
+    end function chf_statistics
+
     integer(c_int) function chf_dims(ctx, mx, mz, ny) bind(C, name="chf_dims")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -55,14 +55,11 @@ program poiseuilleflow_gpu
   call system_clock(c0, crate)
   do istep = istepbegin, nstep                  ! PF:220
     if (istep /= istepbegin) then
+      if (mod(istep, outputstep) == 0) call mean_statistics(istep)   ! statistics, PF:223
       call chf_check(chf_nonlinear(ctx), ctx, 'chf_nonlinear')   ! PF:224
       if (mod(istep, interupt) == 0) then        ! output, PF:225 / 2465
         call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
         call write_restart(istep)
-      end if
-      if (mod(istep, outputstep) == 0) then
-        if (mod(istep, interupt) /= 0) call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
-        call mean_statistics(istep)
       end if
     end if
     call chf_check(chf_solve(ctx), ctx, 'chf_solve')            ! PF:242-474
@@ -120,25 +117,82 @@ contains
     call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
   end subroutine laminar_start
 
-  ! Mean-flow quantities of statistics (PF:3170-3190) from the mean mode:
-  ! wall shear dU/dy(+-1) = sum_k a_k k^2 (+-1)^(k+1), bulk velocity
-  ! (1/2) int U dy = sum_{k even} a_k / (1 - k^2).
+  ! The printed diagnostics of statistics (PF:2816-2868, 3080-3173) from the
+  ! plane statistics reduced on the GPU: CFL numbers, wall shear velocities
+  ! from the total shear stress -<u'v'> + dU/dy / re at the walls, Re_tau,
+  ! Re_c and Re_b.
   subroutine mean_statistics(istep)
     integer, intent(in) :: istep
-    real*8 :: tau_top, tau_bot, ub, s
+    real*8 :: ps(0:nz, 12), dudy(0:nz), tau(0:nz), zz(0:nz), dy(0:nz)
+    real*8 :: pi, dx, ddz, ddy, cfl_x, cfl_y, cfl_z, ubulk, ut_b, ut_t
     integer :: k
-    tau_top = 0.0d0
-    tau_bot = 0.0d0
-    ub = 0.0d0
-    do k = 0, nz
-      s = merge(1.0d0, -1.0d0, mod(k + 1, 2) == 0)
-      tau_top = tau_top + au0(1, k) * dble(k) ** 2
-      tau_bot = tau_bot + au0(1, k) * dble(k) ** 2 * s
-      if (mod(k, 2) == 0) ub = ub + au0(1, k) / (1.0d0 - dble(k) ** 2)
+    call chf_check(chf_statistics(ctx, ps), ctx, 'chf_statistics')
+    pi = 4.0d0 * atan(1.0d0)
+    dx = aphi * 2.0d0 * pi / dble(nnx)
+    ddz = beta * 2.0d0 * pi / dble(nny)
+    ddy = pi / dble(nz)
+    dy = 0.0d0
+    do k = 1, nz / 2
+      dy(k) = -(cos(dble(k) * ddy) - cos(dble(k - 1) * ddy))
+      dy(nz - k) = dy(k)
     end do
-    write(*, '(a,i8,a,es14.6,a,es14.6,a,es14.6,a,es14.6)') 'istep ', istep, &
-        '  dU/dy(+1) ', tau_top, '  dU/dy(-1) ', tau_bot, '  U_bulk ', ub, &
-        '  -dp/dx ', re_p
+    do k = 0, nz
+      zz(k) = cos(dble(k) * ddy)
+    end do
+    cfl_x = maxval(ps(:, 10)) * dt / dx
+    cfl_z = maxval(ps(:, 12)) * dt / ddz
+    cfl_y = 0.0d0
+    do k = 1, nz - 1
+      cfl_y = max(cfl_y, ps(k, 11) * dt / dy(k))
+    end do
+    call derivative_at_points(ps(:, 1), dudy)
+    tau = -ps(:, 7) + dudy / re
+    ut_b = sqrt(abs(tau(nz)))
+    ut_t = sqrt(abs(tau(0)))
+    ubulk = 0.0d0
+    do k = nz / 2, nz - 1
+      ubulk = ubulk + ps(k, 1) * abs(zz(k + 1) - zz(k))
+    end do
+    write(*, '(a,i8)') 'istep=', istep
+    write(*, '(a,es14.6,a,es14.6,a,es14.6)') 'CFL_X= ', cfl_x, ' CFL_Y= ', cfl_y, ' CFL_Z= ', cfl_z
+    write(*, '(a,2es14.6)') 'wall shear velocity: ', ut_b, ut_t
+    write(*, '(a,es14.6,a,2es14.6)') 'Mean pressure gradient[d<p>/dx]: ', -re_p, &
+        ' & The shear-stress gradient: ', -tau(nz), tau(0)
+    write(*, '(a,2es14.6)') 'Reynolds number R_t: ', ut_b * re, ut_t * re
+    write(*, '(a,es14.6)') 'Reynolds number R_c: ', ps(nz / 2, 1) * re
+    write(*, '(a,es14.6)') 'Reynolds number R_b: ', 2.0d0 * ubulk * re
   end subroutine mean_statistics
+
+  ! d/dy of the degree-nz interpolant of f(y_k) at y_k = cos(pi k/nz): the
+  ! chebyshev(-1) -> dz -> chebyshev(1) sequence of PF:3083-3085.
+  subroutine derivative_at_points(f, df)
+    real*8, intent(in) :: f(0:nz)
+    real*8, intent(out) :: df(0:nz)
+    real*8 :: a(0:nz), b(0:nz + 2), pi, w, ck
+    integer :: j, l
+    pi = 4.0d0 * atan(1.0d0)
+    do l = 0, nz
+      a(l) = 0.0d0
+      do j = 0, nz
+        w = 1.0d0
+        if (j == 0 .or. j == nz) w = 0.5d0
+        a(l) = a(l) + w * f(j) * cos(pi * dble(j) * dble(l) / dble(nz))
+      end do
+      ck = 1.0d0
+      if (l == 0 .or. l == nz) ck = 2.0d0
+      a(l) = a(l) * 2.0d0 / (dble(nz) * ck)
+    end do
+    b = 0.0d0
+    do j = nz - 1, 0, -1
+      b(j) = b(j + 2) + 2.0d0 * dble(j + 1) * a(j + 1)
+    end do
+    b(0) = 0.5d0 * b(0)
+    do j = 0, nz
+      df(j) = 0.0d0
+      do l = 0, nz
+        df(j) = df(j) + b(l) * cos(pi * dble(j) * dble(l) / dble(nz))
+      end do
+    end do
+  end subroutine derivative_at_points
 
 end program poiseuilleflow_gpu

@@ -67,12 +67,13 @@ def test_driver_ten_steps_match_oracle(require_gpu, tmp_path):
     outdir = tmp_path / "out"
     outdir.mkdir()
     nml = tmp_path / "run.nml"
-    namelist(nml, cfg, istepbegin=0, nstep=10, interupt=10, isign=1, precision=0,
+    namelist(nml, cfg, istepbegin=0, nstep=10, interupt=10, outputstep=5, isign=1, precision=0,
              restart_in=str(rin), output_dir=str(outdir))
     print("driver run:", DRIVER, nml, flush=True)
     r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
+    assert r.stdout.count("wall shear velocity") == 2   # statistics at steps 5 and 10
     got = restart.read_restart(str(outdir / restart.restart_name(0)), cfg["nnx"], cfg["nny"], cfg["nnz"])
     O.Stepper(p).step(st, 10)
     errs = state_errors(got, st)
