@@ -56,7 +56,8 @@ def kernel_work(d, precision="fp64"):
     ne, no, Ny, Ph, Pn = d["ne"], d["no"], d["Ny"], d["Ph"], d["Pn"]
     rv_rows = (ne - 2) + (no - 2)
     ro_rows = (ne - 1) + (no - 1)
-    inv_bytes = 2 * d["ngroups"] * (ne * ne + no * no) * 8 - (ne * ne + no * no) * 8
+    inv_v_bytes = (d["ngroups"] - 1) * (ne * ne + no * no) * 8   # group 0 has no v system
+    ws_flops = 2.0 * col * ((ne - 1) * (ne - 1) + ne * (ne - 1) + (no - 1) * (no - 1) + no * (no - 1))
     return {
         "synth_padded": (6 * (S + Y), 2.0 * Ph * Ny * col * 6),
         "zfft_inv": (6 * (Y + Z), 0.0),
@@ -71,7 +72,9 @@ def kernel_work(d, precision="fp64"):
                   2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
         "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S,
                   2.0 * ((ne - 1) * ne + (no - 1) * no) * col),
-        "solve": (inv_bytes + (rv_rows + ro_rows) / Ny * S + 2 * S, 0.0),
+        # v solve (omega_y of the mean group only) and the diagonalised omega_y solve
+        "solve": (inv_v_bytes + rv_rows / Ny * S + S, 0.0),
+        "wsolve": (ro_rows / Ny * S + S, ws_flops),
         "kinematics": (6 * S, 0.0),
     }
 
@@ -99,7 +102,7 @@ KERNEL_SYMBOL = {"synth_padded": "k_mfma_cos", "zfft_inv": "k_zfft", "xpass": "k
                  "zfft_fwd": "k_zfft", "analysis": "k_mfma_cos", "hassemble": "k_hassemble",
                  "synth_points": "k_mfma_cos", "mean_points": "k_mean_points",
                  "rhs_prep": "k_rhs_prep", "rhs_v": "k_mfma_cos", "rhs_o": "k_mfma_cos",
-                 "solve": "k_solve", "kinematics": "k_kinematics"}
+                 "solve": "k_solve", "kinematics": "k_kinematics", "wsolve": "k_wsolve"}
 
 
 def measured_traffic(config, form, kernel):

@@ -7,13 +7,14 @@
 #include <vector>
 
 #include "chf_internal.h"
+#include "eig.h"
 
 using namespace chf;
 
 namespace chf {
 const char* const kKernelNames[K_COUNT] = {
     "synth_padded", "zfft_inv", "xpass", "zfft_fwd", "analysis", "hassemble",
-    "synth_points", "mean_points", "rhs_v", "rhs_o", "solve", "rhs_prep", "kinematics"};
+    "synth_points", "mean_points", "rhs_v", "rhs_o", "solve", "rhs_prep", "kinematics", "wsolve"};
 }
 
 namespace {
@@ -148,12 +149,120 @@ void free_all(Ctx* c) {
   f(c->hvc); f(c->hgc);
   f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
-  f(c->inv);
+  f(c->inv); f(c->inv0);
+  for (int i = 0; i < 2; ++i) { f(c->wL[i]); f(c->wR[i]); f(c->wlam[i]); }
   f(c->phys_c); f(c->phys_r[0]); f(c->phys_r[1]); f(c->phys_r[2]); f(c->stats); f(c->twxu); f(c->twzu); f(c->twx); f(c->twz); f(c->costab);
   f(c->gv); f(c->go);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
   if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Matrix-core A fragments of a rows x cols row-major matrix:
+// out[(rb * KB + kb) * 64 + lane] = A[16 rb + lane % 16][4 kb + lane / 16], zero outside.
+std::vector<double> pack_fragments(const std::vector<double>& a, int rows, int cols) {
+  const int MB = (rows + 15) / 16, KB = (cols + 3) / 4;
+  std::vector<double> out((size_t)MB * KB * 64, 0.0);
+  for (int rb = 0; rb < MB; ++rb)
+    for (int kb = 0; kb < KB; ++kb)
+      for (int l = 0; l < 64; ++l) {
+        const int r = 16 * rb + l % 16, k = 4 * kb + l / 16;
+        if (r < rows && k < cols) out[((size_t)rb * KB + kb) * 64 + l] = a[(size_t)r * cols + k];
+      }
+  return out;
+}
+
+// The omega_y half systems (get_coef_omegay, PF:3432-3455, folded by parity)
+// are [1^T ; c (sigma T0_I - T2_I)] x = [0 ; b] with c = dt/(2 re) and
+// sigma = 1/c + k^2: k^2 enters through sigma only.  On the null space of the
+// wall row, x = Q z with Q = [-1^T ; I], the interior rows read
+// c (sigma M0 - M2) z = b with M0 = T0_I Q, M2 = T2_I Q.  With
+// M0^-1 M2 = V diag(lam) V^-1 (lam: the Dirichlet second-derivative
+// eigenvalues, real and distinct),
+//     x = (Q V) diag(1 / (1 + c (k^2 - lam))) (V^-1 M0^-1) b,
+// so every mode shares two matrices and the per-mode work is a diagonal
+// scale.  The decomposition is accepted only if it reproduces direct solves
+// of the half system at k^2 = 0, 1 and the largest k^2 to 1e-12.
+int omega_diagonalise(Ctx* c, const std::vector<double>& t0, const std::vector<double>& t2,
+                      const std::vector<double>& al, const std::vector<double>& be) {
+  const Dims& d = c->d;
+  const int Ny = d.Ny;
+  const double cc = c->prm.dt * 0.5 / c->prm.re;
+  double smax = 0.0;
+  for (double a : al)
+    for (double b : be) smax = std::max(smax, a * a + b * b);
+  std::vector<double> Lh[2], Rh[2], lam[2];
+  for (int par = 0; par < 2; ++par) {
+    const int n = par ? d.no : d.ne, m = n - 1;
+    if (m < 1) return 0;
+    la::Mat M0((size_t)m * m), M2((size_t)m * m);
+    for (int r = 0; r < m; ++r) {
+      const int p = r + 1;  // interior point of row r
+      for (int k = 0; k < m; ++k) {
+        const int l0 = par, lk = 2 * (k + 1) + par;  // Q column k: -1 at x_0, +1 at x_{k+1}
+        M0[(size_t)r * m + k] = t0[(size_t)p * Ny + lk] - t0[(size_t)p * Ny + l0];
+        M2[(size_t)r * m + k] = t2[(size_t)p * Ny + lk] - t2[(size_t)p * Ny + l0];
+      }
+    }
+    la::Mat M0i, V, Vi;
+    if (!la::inverse(M0, m, M0i)) return 0;
+    const la::Mat G = la::matmul(M0i, M2, m, m, m);
+    std::vector<double> ev;
+    if (!la::eig_real(G, m, ev, V)) return 0;
+    if (!la::inverse(V, m, Vi)) return 0;
+    la::Mat L = la::matmul(Vi, M0i, m, m, m);
+    la::Mat R((size_t)n * m);
+    for (int k = 0; k < m; ++k) {
+      double s = 0.0;
+      for (int r = 0; r < m; ++r) s += V[(size_t)r * m + k];
+      R[k] = -s;
+      for (int r = 0; r < m; ++r) R[(size_t)(r + 1) * m + k] = V[(size_t)r * m + k];
+    }
+    // verification against direct solves of the half system
+    for (double s : {0.0, 1.0, smax}) {
+      la::Mat A((size_t)n * n);
+      for (int j = 0; j < n; ++j) A[j] = 1.0;
+      for (int i = 1; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+          const size_t o = (size_t)i * Ny + 2 * j + par;
+          A[(size_t)i * n + j] = -cc * t2[o] + (1.0 + cc * s) * t0[o];
+        }
+      std::vector<double> b(m), x(n, 0.0), y(m, 0.0);
+      for (int r = 0; r < m; ++r) b[r] = std::sin(1.7 * r + 0.3) + 0.1 * r / m;
+      la::Mat rhs(n, 0.0);
+      for (int r = 0; r < m; ++r) rhs[r + 1] = b[r];
+      std::vector<int> piv;
+      if (!la::lu(A, piv, n)) return 0;
+      la::lu_solve(A, piv, n, rhs, 1);
+      for (int j = 0; j < m; ++j) {
+        double t = 0.0;
+        for (int r = 0; r < m; ++r) t += L[(size_t)j * m + r] * b[r];
+        y[j] = t / (1.0 + cc * (s - ev[j]));
+      }
+      double err = 0.0, ref = 0.0;
+      for (int i = 0; i < n; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < m; ++j) t += R[(size_t)i * m + j] * y[j];
+        err = std::max(err, std::fabs(t - rhs[i]));
+        ref = std::max(ref, std::fabs(rhs[i]));
+      }
+      if (!(err <= 1e-12 * ref)) return 0;
+    }
+    Lh[par] = pack_fragments(L, m, m);
+    Rh[par] = pack_fragments(R, n, m);
+    lam[par] = ev;
+    lam[par].resize((size_t)((m + 15) / 16) * 16, 0.0);
+  }
+  for (int par = 0; par < 2; ++par) {
+    CHF_TRY(dalloc(c, &c->wL[par], Lh[par].size()));
+    CHF_TRY(dalloc(c, &c->wR[par], Rh[par].size()));
+    CHF_TRY(dalloc(c, &c->wlam[par], lam[par].size()));
+    CHF_TRY(hipMemcpy(c->wL[par], Lh[par].data(), Lh[par].size() * sizeof(double), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->wR[par], Rh[par].data(), Rh[par].size() * sizeof(double), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->wlam[par], lam[par].data(), lam[par].size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  c->wdiag = true;
+  return 0;
 }
 
 int build(Ctx* c) {
@@ -172,8 +281,6 @@ int build(Ctx* c) {
   d.Pn = d.N / 2 + 1;
   d.ncol = (long)d.Mz * d.Mx;
   d.ngroups = d.Mx * (d.Mz / 2);
-  d.inv_row = 2L * ((d.ne + (d.ne & 1)) + (d.no + (d.no & 1)));
-  d.inv_stride = (long)d.ne * d.inv_row;
 
   CHF_TRY(hipGetDevice(&c->device));
   int max_lds = 0;
@@ -223,7 +330,6 @@ int build(Ctx* c) {
   CHF_TRY(dalloc(c, &c->roO, (size_t)d.ne * d.ncol));
   CHF_TRY(dalloc(c, &c->gv, fld));
   CHF_TRY(dalloc(c, &c->go, fld));
-  CHF_TRY(dalloc(c, &c->inv, (size_t)d.ngroups * d.inv_stride));
 
   // wavenumbers (PF:123-134, id = 0)
   std::vector<double> al(d.Mx), be(d.Mz);
@@ -242,6 +348,30 @@ int build(Ctx* c) {
   CHF_TRY(hipMemcpy(c->t0, t0.data(), tt * sizeof(double), hipMemcpyHostToDevice));
   CHF_TRY(hipMemcpy(c->t2, t2.data(), tt * sizeof(double), hipMemcpyHostToDevice));
   CHF_TRY(hipMemcpy(c->t4, t4.data(), tt * sizeof(double), hipMemcpyHostToDevice));
+
+  // omega_y operator: diagonalised when the decomposition verifies, else its
+  // half-system inverses are stored next to the v ones
+  {
+    const int rc = omega_diagonalise(c, t0, t2, al, be);
+    if (rc) return rc;
+    const long neP = d.ne + (d.ne & 1), noP = d.no + (d.no & 1);
+    if (c->wdiag) {
+      d.inv_row = neP + noP;
+      d.inv_stride = (long)d.ne * d.inv_row;
+      CHF_TRY(dalloc(c, &c->inv, (size_t)d.ngroups * d.inv_stride));
+      CHF_TRY(dalloc(c, &c->inv0, (size_t)d.ne * (neP + noP)));
+      c->inv_w = c->inv0;
+      c->stride_w = 0;
+      c->row_w = neP + noP;
+    } else {
+      d.inv_row = 2 * (neP + noP);
+      d.inv_stride = (long)d.ne * d.inv_row;
+      CHF_TRY(dalloc(c, &c->inv, (size_t)d.ngroups * d.inv_stride));
+      c->inv_w = c->inv + (neP + noP);
+      c->stride_w = d.inv_stride;
+      c->row_w = d.inv_row;
+    }
+  }
 
   // transform matrices: synthesis on the 2N+1 grid (folded rows p = 0..N),
   // analysis weights (PF:1696-1700: /2N, c_0 = 2, end points weighted once),
@@ -373,6 +503,7 @@ int solve(Ctx* c) {
   CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
   CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
   CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c); }));
+  if (c->wdiag) CHF_TRY(timed(c, K_WSOLVE, [&] { return launch_wsolve(*c); }));
   CHF_TRY(timed(c, K_KINEMATICS, [&] { return launch_kinematics(*c); }));
   return 0;
 }

@@ -101,6 +101,18 @@ struct Ctx {
   // so a solve block reads one contiguous row per step.  Group 0: mean mode
   // (omega_y-type operator at k^2 = 0), its v halves are zero.
   double* inv = nullptr;
+  // omega_y half systems: either their inverses sit beside the v ones in `inv`
+  // (inv_w = inv + neP + noP, same strides), or the operator is diagonalised
+  // (wdiag) and only the mean group's k^2 = 0 inverse is stored, in inv0.
+  bool wdiag = false;
+  double* inv0 = nullptr;
+  const double* inv_w = nullptr;
+  long stride_w = 0, row_w = 0;
+  // diagonalised omega_y solve, per parity: packed matrix-core A fragments of
+  // L = V^-1 M0^-1 ((n-1) x (n-1)) and R = Q V (n x (n-1)), eigenvalues lam
+  double* wL[2] = {nullptr, nullptr};
+  double* wR[2] = {nullptr, nullptr};
+  double* wlam[2] = {nullptr, nullptr};
   double2* twx = nullptr;  // exp(-2 pi i m/Lx), m < Lx
   double2* twz = nullptr;  // exp(-2 pi i m/Lz), m < Lz
   // unpadded physical output and plane statistics (allocated on first use)
@@ -135,7 +147,7 @@ bool fft_length_ok(int L);
 // Kernel classes for per-launch timing.
 enum KernelId {
   K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
-  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_KINEMATICS, K_COUNT
+  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_KINEMATICS, K_WSOLVE, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -160,6 +172,7 @@ hipError_t launch_mean_points(Ctx& c);
 hipError_t launch_rhs_prep(Ctx& c);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
 hipError_t launch_solve(Ctx& c);       // v, omega_y and the mean mode
+hipError_t launch_wsolve(Ctx& c);      // diagonalised omega_y solve (all modes but the mean)
 hipError_t launch_kinematics(Ctx& c);  // u, w, omega_x, omega_z from v, omega_y, U, W
 hipError_t launch_setup_inverses(Ctx& c);
 

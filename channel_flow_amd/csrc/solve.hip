@@ -14,6 +14,7 @@
 // LDS); a step multiplies the stored inverses, as the reference multiplies the
 // zgetri inverses with a_multiply_x (PF:4123-4139).
 #include "chf_internal.h"
+#include "mfma.h"
 
 namespace chf {
 
@@ -22,9 +23,11 @@ namespace chf {
 struct SetupArgs {
   const double *t0, *t2, *t4;
   const double *al, *be;
-  double* inv;
+  double* inv;      // v halves (and omega halves when not diagonalised)
+  double* inv_w;    // omega halves
   int Mx, N, Ny, ne, no;
-  long stride, row;
+  long stride, row, stride_w, row_w;
+  int wdiag;        // omega operator diagonalised: only group 0's omega halves are stored
   double dt, re;
 };
 
@@ -52,11 +55,13 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
   const double k2 = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
   const int n = par ? a.no : a.ne;
   const int nP = n + (n & 1);  // even width: 16-byte aligned row pairs
-  const int neP = a.ne + (a.ne & 1), noP = a.no + (a.no & 1);
-  double* out = a.inv + (long)g * a.stride + eq * (neP + noP) + par * neP;
+  const int neP = a.ne + (a.ne & 1);
+  if (eq == 1 && g > 0 && a.wdiag) return;
+  const long orow = eq ? a.row_w : a.row;
+  double* out = (eq ? a.inv_w + (long)g * a.stride_w : a.inv + (long)g * a.stride) + par * neP;
   if (eq == 0 && g == 0) {  // k^2 = 0 has no v system (PF:184, 247)
     for (int idx = threadIdx.x; idx < a.ne * nP; idx += blockDim.x)
-      out[(long)(idx / nP) * a.row + idx % nP] = 0.0;
+      out[(long)(idx / nP) * orow + idx % nP] = 0.0;
     return;
   }
   double* A = sm;  // n x n row-major
@@ -129,7 +134,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
   // transposed store: row j, entry i = Ainv[i][j]; zero padding for i >= n and j >= n
   for (int idx = threadIdx.x; idx < a.ne * nP; idx += blockDim.x) {
     const int j = idx / nP, i = idx - j * nP;
-    out[(long)j * a.row + i] = (i < n && j < n) ? A[i * n + j] : 0.0;
+    out[(long)j * orow + i] = (i < n && j < n) ? A[i * n + j] : 0.0;
   }
 }
 
@@ -142,6 +147,7 @@ hipError_t launch_setup_inverses(Ctx& c) {
   a.al = c.alpha;
   a.be = c.beta;
   a.inv = c.inv;
+  a.inv_w = const_cast<double*>(c.inv_w);
   a.Mx = d.Mx;
   a.N = d.N;
   a.Ny = d.Ny;
@@ -149,6 +155,9 @@ hipError_t launch_setup_inverses(Ctx& c) {
   a.no = d.no;
   a.stride = d.inv_stride;
   a.row = d.inv_row;
+  a.stride_w = c.stride_w;
+  a.row_w = c.row_w;
+  a.wdiag = c.wdiag ? 1 : 0;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
   size_t shm = sizeof(double) * (size_t)d.ne * d.ne;
@@ -165,12 +174,14 @@ hipError_t launch_setup_inverses(Ctx& c) {
 struct SolveArgs {
   const double2 *rvE, *rvO, *roE, *roO;  // parity-folded right-hand sides
   const double* inv;
+  const double* inv_w;
   double2 *v, *oy;
   double2 *au0, *w0;
   const double2 *c1n, *c1p, *c3n, *c3p;  // mean Lamb point values, n and n-1
   const double *t0, *t2;
   int Mx, Mz, N, Ny, ne, no;
-  long stride, row, ncol;
+  long stride, row, stride_w, row_w, ncol;
+  int wdiag;
   double dt, re;
 };
 
@@ -267,12 +278,13 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
   for (int t = threadIdx.x; t < 2 * (pe + po); t += blockDim.x) {
     const int eq = t / (pe + po);
     if (mean && eq == 0) continue;
+    if (!mean && eq == 1 && a.wdiag) continue;  // omega_y by the diagonalised solve
     const int rr = t - eq * (pe + po);
     const bool odd = rr >= pe;
     const int i2 = 2 * (odd ? rr - pe : rr);
     const int n = odd ? no : ne;
-    const long nP = a.row;
-    const double* M = a.inv + (long)g * a.stride + eq * (neP + noP) + (odd ? neP : 0) + i2;
+    const long nP = eq ? a.row_w : a.row;
+    const double* M = (eq ? a.inv_w + (long)g * a.stride_w : a.inv + (long)g * a.stride) + (odd ? neP : 0) + i2;
     const double* rhs = rb + eq * 4 * Ny + (odd ? ne : 0);
     double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int j = 0;
@@ -438,6 +450,7 @@ hipError_t launch_solve(Ctx& c) {
   a.roE = c.roE;
   a.roO = c.roO;
   a.inv = c.inv;
+  a.inv_w = c.inv_w;
   a.v = c.v;
   a.oy = c.oy;
   a.au0 = c.au0;
@@ -456,6 +469,9 @@ hipError_t launch_solve(Ctx& c) {
   a.no = d.no;
   a.stride = d.inv_stride;
   a.row = d.inv_row;
+  a.stride_w = c.stride_w;
+  a.row_w = c.row_w;
+  a.wdiag = c.wdiag ? 1 : 0;
   a.ncol = d.ncol;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
@@ -465,6 +481,161 @@ hipError_t launch_solve(Ctx& c) {
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_solve, dim3(d.ngroups), dim3(256), shm, c.stream, a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------- diagonalised omega_y solve
+//
+// For every mode but the mean, omega_y^{n+1} of parity p is
+//   x = R diag(1 / (1 + c (k^2 - lam))) L b
+// (L, R, lam from the set-up), i.e. two FP64 matrix-core products with
+// matrices shared by all modes.  Block = one parity and 64 real columns (32
+// complex modes, kx fastest): the right-hand sides b (the n-1 interior rows)
+// are staged in LDS, Y = L B runs on v_mfma_f64_16x16x4f64 with the A
+// fragments of L read from L2, Y is scaled per (row, mode) and written back
+// to LDS as the B operand of X = R Y, and X lands in omega_y at degrees 2i + p.
+struct WArgs {
+  const double *L0, *L1, *R0, *R1, *lam0, *lam1;
+  const double *ro0, *ro1;  // parity-folded right-hand sides, real view [rows][2 ncol]
+  double* oy;               // real view [Ny][2 ncol]
+  const double *al, *be;
+  int n0, n1, Mx, Mz;
+  long ncold;
+  double c;
+};
+
+__global__ __launch_bounds__(256) void k_wsolve(WArgs a) {
+  using MF = Mf<double>;
+  constexpr int CT = 64, TPW = 3;  // columns per block, row blocks per wave (<= 12 row blocks)
+  extern __shared__ double sB[];   // [rows][CT]
+  const int par = blockIdx.y;
+  const int n = par ? a.n1 : a.n0, m = n - 1;
+  const double* L = par ? a.L1 : a.L0;
+  const double* R = par ? a.R1 : a.R0;
+  const double* lam = par ? a.lam1 : a.lam0;
+  const double* ro = par ? a.ro1 : a.ro0;
+  const int MB1 = (m + 15) / 16, KB = (m + 3) / 4, MB2 = (n + 15) / 16;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long col0 = (long)blockIdx.x * CT;
+  // stage B: rows r < m, zero rows up to 16 MB1
+  for (int idx = threadIdx.x; idx < 16 * MB1 * CT; idx += blockDim.x) {
+    const int r = idx / CT, cc = idx - r * CT;
+    const long col = col0 + cc;
+    sB[idx] = (r < m && col < a.ncold) ? ro[(long)r * a.ncold + col] : 0.0;
+  }
+  __syncthreads();
+  MF::acc acc[TPW][4];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::acc{0, 0, 0, 0};
+  for (int kb = 0; kb < KB; ++kb) {
+    double bf[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) bf[cb] = sB[(4 * kb + (lane >> 4)) * CT + 16 * cb + (lane & 15)];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int rb = wave + 4 * t;
+      if (rb < MB1) {
+        const double af = L[((long)rb * KB + kb) * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::mma(af, bf[cb], acc[t][cb]);
+      }
+    }
+  }
+  __syncthreads();
+  // scale rows by 1 / (1 + c (k^2 - lam)) and write Y back as the next B operand
+  double s[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const long cm = (col0 + 16 * cb + (lane & 15)) >> 1;
+    const int kz = (int)(cm / a.Mx), kx = (int)(cm - (long)kz * a.Mx);
+    const bool in = cm < (a.ncold >> 1);
+    const double alx = in ? a.al[kx] : 0.0, bez = in ? a.be[kz] : 0.0;
+    s[cb] = alx * alx + bez * bez;
+  }
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int rb = wave + 4 * t;
+    if (rb >= MB1) continue;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = 16 * rb + MF::row(lane, e);
+      const double lr = lam[row];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const double y = row < m ? acc[t][cb][e] / (1.0 + a.c * (s[cb] - lr)) : 0.0;
+        sB[row * CT + 16 * cb + (lane & 15)] = y;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::acc{0, 0, 0, 0};
+  for (int kb = 0; kb < KB; ++kb) {
+    double bf[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) bf[cb] = sB[(4 * kb + (lane >> 4)) * CT + 16 * cb + (lane & 15)];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int rb = wave + 4 * t;
+      if (rb < MB2) {
+        const double af = R[((long)rb * KB + kb) * 64 + lane];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::mma(af, bf[cb], acc[t][cb]);
+      }
+    }
+  }
+  // store x_i -> omega_y at degree 2i + p; the mean mode and the span Nyquist
+  // column are not solved here (PF:247)
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const long col = col0 + 16 * cb + (lane & 15);
+    const long cm = col >> 1;
+    if (col >= a.ncold || cm == 0) continue;
+    const int kz = (int)(cm / a.Mx);
+    if (kz == a.Mz / 2) continue;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int rb = wave + 4 * t;
+      if (rb >= MB2) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = 16 * rb + MF::row(lane, e);
+        if (i < n) a.oy[(long)(2 * i + par) * a.ncold + col] = acc[t][cb][e];
+      }
+    }
+  }
+}
+
+hipError_t launch_wsolve(Ctx& c) {
+  const Dims& d = c.d;
+  WArgs a{};
+  a.L0 = c.wL[0];
+  a.L1 = c.wL[1];
+  a.R0 = c.wR[0];
+  a.R1 = c.wR[1];
+  a.lam0 = c.wlam[0];
+  a.lam1 = c.wlam[1];
+  a.ro0 = reinterpret_cast<const double*>(c.roE);
+  a.ro1 = reinterpret_cast<const double*>(c.roO);
+  a.oy = reinterpret_cast<double*>(c.oy);
+  a.al = c.alpha;
+  a.be = c.beta;
+  a.n0 = d.ne;
+  a.n1 = d.no;
+  a.Mx = d.Mx;
+  a.Mz = d.Mz;
+  a.ncold = 2 * d.ncol;
+  a.c = c.prm.dt * 0.5 / c.prm.re;
+  const int m = d.ne - 1;
+  const size_t shm = sizeof(double) * 64 * 16 * ((m + 15) / 16);
+  if (shm > 65536) return hipErrorInvalidValue;
+  if ((d.ne + 15) / 16 > 12) return hipErrorInvalidValue;
+  const unsigned tiles = (unsigned)((a.ncold + 63) / 64);
+  hipLaunchKernelGGL(k_wsolve, dim3(tiles, 2), dim3(256), shm, c.stream, a);
   return hipGetLastError();
 }
 
