@@ -480,7 +480,9 @@ hipError_t launch_solve(Ctx& c) {
     hipError_t e = hipFuncSetAttribute((const void*)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_solve, dim3(d.ngroups), dim3(256), shm, c.stream, a);
+  // with the omega_y operator diagonalised a block streams the two v half
+  // systems only: ne/2 + no/2 row pairs, so 128 threads
+  hipLaunchKernelGGL(k_solve, dim3(d.ngroups), dim3(c.wdiag ? 128 : 256), shm, c.stream, a);
   return hipGetLastError();
 }
 
@@ -494,6 +496,8 @@ hipError_t launch_solve(Ctx& c) {
 // are staged in LDS, Y = L B runs on v_mfma_f64_16x16x4f64 with the A
 // fragments of L read from L2, Y is scaled per (row, mode) and written back
 // to LDS as the B operand of X = R Y, and X lands in omega_y at degrees 2i + p.
+// Eight wavefronts per block, each owning up to two 16-row blocks, keep enough
+// independent fragment loads in flight to cover the L2 latency.
 struct WArgs {
   const double *L0, *L1, *R0, *R1, *lam0, *lam1;
   const double *ro0, *ro1;  // parity-folded right-hand sides, real view [rows][2 ncol]
@@ -504,9 +508,9 @@ struct WArgs {
   double c;
 };
 
-__global__ __launch_bounds__(256) void k_wsolve(WArgs a) {
+__global__ __launch_bounds__(512) void k_wsolve(WArgs a) {
   using MF = Mf<double>;
-  constexpr int CT = 64, TPW = 3;  // columns per block, row blocks per wave (<= 12 row blocks)
+  constexpr int CT = 64, NW = 8, TPW = 2;  // columns, waves, row blocks per wave (<= 16 row blocks)
   extern __shared__ double sB[];   // [rows][CT]
   const int par = blockIdx.y;
   const int n = par ? a.n1 : a.n0, m = n - 1;
@@ -535,7 +539,7 @@ __global__ __launch_bounds__(256) void k_wsolve(WArgs a) {
     for (int cb = 0; cb < 4; ++cb) bf[cb] = sB[(4 * kb + (lane >> 4)) * CT + 16 * cb + (lane & 15)];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-      const int rb = wave + 4 * t;
+      const int rb = wave + NW * t;
       if (rb < MB1) {
         const double af = L[((long)rb * KB + kb) * 64 + lane];
 #pragma unroll
@@ -556,7 +560,7 @@ __global__ __launch_bounds__(256) void k_wsolve(WArgs a) {
   }
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
-    const int rb = wave + 4 * t;
+    const int rb = wave + NW * t;
     if (rb >= MB1) continue;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -580,7 +584,7 @@ __global__ __launch_bounds__(256) void k_wsolve(WArgs a) {
     for (int cb = 0; cb < 4; ++cb) bf[cb] = sB[(4 * kb + (lane >> 4)) * CT + 16 * cb + (lane & 15)];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-      const int rb = wave + 4 * t;
+      const int rb = wave + NW * t;
       if (rb < MB2) {
         const double af = R[((long)rb * KB + kb) * 64 + lane];
 #pragma unroll
@@ -599,7 +603,7 @@ __global__ __launch_bounds__(256) void k_wsolve(WArgs a) {
     if (kz == a.Mz / 2) continue;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-      const int rb = wave + 4 * t;
+      const int rb = wave + NW * t;
       if (rb >= MB2) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -633,9 +637,9 @@ hipError_t launch_wsolve(Ctx& c) {
   const int m = d.ne - 1;
   const size_t shm = sizeof(double) * 64 * 16 * ((m + 15) / 16);
   if (shm > 65536) return hipErrorInvalidValue;
-  if ((d.ne + 15) / 16 > 12) return hipErrorInvalidValue;
+  if ((d.ne + 15) / 16 > 16) return hipErrorInvalidValue;
   const unsigned tiles = (unsigned)((a.ncold + 63) / 64);
-  hipLaunchKernelGGL(k_wsolve, dim3(tiles, 2), dim3(256), shm, c.stream, a);
+  hipLaunchKernelGGL(k_wsolve, dim3(tiles, 2), dim3(512), shm, c.stream, a);
   return hipGetLastError();
 }
 
