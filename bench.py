@@ -241,6 +241,7 @@ def run_form(cfg, ic_arrays, precision, steps, warmup, total, log):
                  cfg["re_p"], precision=precision)
     t_setup = time.perf_counter() - t0
     log(f"{precision}: setup {t_setup:.2f}s, device bytes {ch.device_bytes() / 1e9:.2f} GB")
+    tiles = ch.tile_choices()
     ch.initial(v, oy, au0, w0)
     ch.step(warmup)
     ch.set_timing(True)
@@ -260,7 +261,7 @@ def run_form(cfg, ic_arrays, precision, steps, warmup, total, log):
     per_kernel = {k: dict(total_ms=round(ms, 3), launches=n, avg_ms=round(ms / n, 4) if n else None)
                   for k, (ms, n) in kt.items() if n}
     return dict(ms_per_step=wall * 1e3 / steps, per_kernel=per_kernel, state=st, fields=fl,
-                setup_s=t_setup)
+                setup_s=t_setup, tiles=tiles)
 
 
 def field_errors(cfg, mixed, ref):
@@ -368,6 +369,7 @@ def main(argv=None):
         "cpu_baseline": cpu,
         "kernels": pk,
         "setup_s": round(r["setup_s"], 2),
+        "autotuned_tiles": r["tiles"],
         "state_finite": bool(finite),
     }
     if "fp64" in runs and "mixed" in runs:

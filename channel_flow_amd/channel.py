@@ -175,6 +175,17 @@ class Channel:
         return statistics.diagnostics(self.plane_statistics(), p.nnx, p.nny, p.nnz, p.aphi, p.beta, p.dt,
                                       p.re, p.re_p)
 
+    def tile_choices(self) -> dict:
+        """Column blocks per wavefront the autotuner picked for each tuned
+        contraction, with the measured ms per launch for 1, 2 and 4."""
+        out = {}
+        for i in range(self._lib.chf_kernel_count()):
+            ms = (ctypes.c_double * 3)()
+            cb = self._lib.chf_tile_choice(self._ctx, i, ms)
+            if cb > 0:
+                out[self._lib.chf_kernel_name(i).decode()] = dict(choice=cb, ms=[round(x, 4) for x in ms])
+        return out
+
     def device_bytes(self) -> int:
         return int(self._lib.chf_device_bytes(self._ctx))
 

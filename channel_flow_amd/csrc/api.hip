@@ -265,6 +265,74 @@ int omega_diagonalise(Ctx* c, const std::vector<double>& t0, const std::vector<d
   return 0;
 }
 
+// Autotuner: for each cos-table contraction, times 1, 2 and 4 column blocks
+// per wave on this context's own shapes and buffers (a warm-up launch, then the
+// mean of three) and keeps the fastest.  The state is all zeros at this point
+// and every choice computes bit-identical outputs, so tuning has no effect on
+// results, only on speed.
+int autotune(Ctx* c) {
+  struct Stage {
+    int id;
+    std::function<hipError_t()> run;
+  };
+  std::vector<Stage> st;
+  if (c->prm.precision == 0) {
+    st.push_back({K_SYNTH_PADDED, [c] {
+                    const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
+                    return launch_synth_padded(*c, in6, c->Y, 6);
+                  }});
+    st.push_back({K_ANALYSIS, [c] {
+                    const double2* y3[3] = {c->Y[0], c->Y[1], c->Y[2]};
+                    return launch_analysis(*c, y3, c->Lc, 3);
+                  }});
+  } else {
+    st.push_back({K_SYNTH_PADDED, [c] {
+                    const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
+                    return launch_synth_padded32(*c, in6, c->Yf, 6);
+                  }});
+    st.push_back({K_ANALYSIS, [c] {
+                    const float2* y3[3] = {c->Yf[0], c->Yf[1], c->Yf[2]};
+                    return launch_analysis32(*c, y3, c->Lc, 3);
+                  }});
+  }
+  st.push_back({K_SYNTH_POINTS, [c] {
+                  const double2* hc[2] = {c->hvc, c->hgc};
+                  double2* ho[2] = {c->hv[0], c->hg[0]};
+                  return launch_synth_points(*c, hc, ho, 2);
+                }});
+  st.push_back({K_RHS_V, [c] { return launch_rhs(*c, 0); }});
+  st.push_back({K_RHS_O, [c] { return launch_rhs(*c, 1); }});
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CHF_TRY(hipEventCreate(&e0));
+  CHF_TRY(hipEventCreate(&e1));
+  const int cand[3] = {1, 2, 4};
+  hipError_t err = hipSuccess;
+  for (auto& s : st) {
+    float best = 3.0e38f;
+    int bestcb = 0;
+    for (int i = 0; i < 3 && err == hipSuccess; ++i) {
+      c->cb_choice[s.id] = cand[i];
+      err = s.run();
+      if (err == hipSuccess) err = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < 3 && err == hipSuccess; ++r) err = s.run();
+      if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      c->tune_ms[s.id][i] = ms / 3.0f;
+      if (err == hipSuccess && ms < best) {
+        best = ms;
+        bestcb = cand[i];
+      }
+    }
+    c->cb_choice[s.id] = bestcb;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CHF_TRY(err);
+  return 0;
+}
+
 int build(Ctx* c) {
   Dims& d = c->d;
   const chf_params& p = c->prm;
@@ -439,7 +507,7 @@ int build(Ctx* c) {
   // kernels that stage more than 64 KiB of LDS
   CHF_TRY(launch_setup_inverses(*c));
   CHF_TRY(hipStreamSynchronize(c->stream));
-  return 0;
+  return autotune(c);
 }
 
 hipEvent_t next_event(Ctx* c) {
@@ -748,6 +816,15 @@ int chf_statistics(void* ctx, double* out) {
                          c->stream));
   CHF_TRY(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int chf_tile_choice(void* ctx, int id, double* ms_by_choice) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (id < 0 || id >= K_COUNT) return fail(c, "chf_tile_choice: kernel id out of range");
+  if (ms_by_choice)
+    for (int i = 0; i < 3; ++i) ms_by_choice[i] = c->tune_ms[id][i];
+  return c->cb_choice[id];
 }
 
 int chf_kernel_count(void) { return K_COUNT; }

@@ -64,7 +64,7 @@ struct SynthP {
 
 template <class TI, class TO, class TC>
 static hipError_t synth_t(Ctx& c, const TC* costab, int f, int Ph, int P, const TI* const* in,
-                          TO* const* out, int nf) {
+                          TO* const* out, int nf, int cb) {
   SynthP<TI, TO, TC> pb{};
   pb.costab = costab;
   pb.rtab = c.costab;
@@ -78,7 +78,7 @@ static hipError_t synth_t(Ctx& c, const TC* costab, int f, int Ph, int P, const 
   pb.Ny = c.d.Ny;
   pb.P = P;
   pb.ncold = 2 * c.d.ncol;
-  return run_mfma_cos(c.stream, pb, nf);
+  return run_mfma_cos(c.stream, pb, nf, cb);
 }
 
 static void as_real(const double2* const* in, const double** out, int nf) {
@@ -90,7 +90,7 @@ hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const*
   double* ro[6];
   as_real(in, ri, nf);
   for (int i = 0; i < nf; ++i) ro[i] = reinterpret_cast<double*>(out[i]);
-  return synth_t<double, double, double>(c, c.costab, 1, c.d.Ph, c.d.Py, ri, ro, nf);
+  return synth_t<double, double, double>(c, c.costab, 1, c.d.Ph, c.d.Py, ri, ro, nf, c.cb_choice[K_SYNTH_PADDED]);
 }
 
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf) {
@@ -98,7 +98,7 @@ hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const*
   double* ro[6];
   as_real(in, ri, nf);
   for (int i = 0; i < nf; ++i) ro[i] = reinterpret_cast<double*>(out[i]);
-  return synth_t<double, double, double>(c, c.costab, 2, c.d.Pn, c.d.Ny, ri, ro, nf);
+  return synth_t<double, double, double>(c, c.costab, 2, c.d.Pn, c.d.Ny, ri, ro, nf, c.cb_choice[K_SYNTH_POINTS]);
 }
 
 // Mixed form: FP64 coefficients -> FP32 values on the padded grid, FP32 matrix cores.
@@ -107,7 +107,7 @@ hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const
   float* ro[6];
   as_real(in, ri, nf);
   for (int i = 0; i < nf; ++i) ro[i] = reinterpret_cast<float*>(out[i]);
-  return synth_t<double, float, float>(c, c.costabf, 1, c.d.Ph, c.d.Py, ri, ro, nf);
+  return synth_t<double, float, float>(c, c.costabf, 1, c.d.Ph, c.d.Py, ri, ro, nf, c.cb_choice[K_SYNTH_PADDED]);
 }
 
 // ----------------------------------------------------------------- analysis
@@ -160,6 +160,7 @@ struct AnalP {
 
 template <class TI, class TC>
 static hipError_t analysis_t(Ctx& c, const TC* costab, const TI* const* in, double2* const* out, int nf) {
+  const int cb = c.cb_choice[K_ANALYSIS];
   AnalP<TI, double, TC> pb{};
   pb.costab = costab;
   pb.rtab = c.costab;
@@ -173,7 +174,7 @@ static hipError_t analysis_t(Ctx& c, const TC* costab, const TI* const* in, doub
   pb.N = c.d.N;
   pb.ncold = 2 * c.d.ncol;
   pb.inv2n = 1.0 / (2.0 * c.d.N);
-  return run_mfma_cos(c.stream, pb, nf);
+  return run_mfma_cos(c.stream, pb, nf, cb);
 }
 
 hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
@@ -329,6 +330,7 @@ hipError_t launch_rhs_prep(Ctx& c) {
 template <int POFF>
 static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const double2* hp, double2* oE,
                           double2* oO) {
+  const int cb = c.cb_choice[POFF == 2 ? K_RHS_V : K_RHS_O];
   RhsSynP<POFF> pb{};
   pb.costab = c.costab;
   pb.n4 = 4 * c.d.N;
@@ -342,7 +344,7 @@ static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const dou
   pb.no = c.d.no;
   pb.ncold = 2 * c.d.ncol;
   pb.dt = c.prm.dt;
-  return run_mfma_cos(c.stream, pb, 1);
+  return run_mfma_cos(c.stream, pb, 1, cb);
 }
 
 hipError_t launch_rhs(Ctx& c, int which) {

@@ -61,6 +61,12 @@ struct ChebOp {
   MfmaShape sh{};
 };
 
+// Kernel classes for per-launch timing.
+enum KernelId {
+  K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
+  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_KINEMATICS, K_WSOLVE, K_COUNT
+};
+
 struct Ctx {
   chf_params prm{};
   Dims d{};
@@ -122,6 +128,10 @@ struct Ctx {
   double2 *twxu = nullptr, *twzu = nullptr;           // unpadded twiddles (2 Mx, Mz)
   long long bytes = 0;
   std::string err;
+  // column blocks per wave of each cos-table contraction, picked at creation
+  // by timing the candidates on the context's own shapes (0 = default)
+  int cb_choice[K_COUNT] = {};
+  float tune_ms[K_COUNT][3] = {};
   // optional per-launch HIP-event timing on the context stream
   bool timing = false;
   struct Mark {
@@ -144,11 +154,6 @@ struct FftPlan {
 FftPlan make_plan(int L);
 bool fft_length_ok(int L);
 
-// Kernel classes for per-launch timing.
-enum KernelId {
-  K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
-  K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_KINEMATICS, K_WSOLVE, K_COUNT
-};
 extern const char* const kKernelNames[K_COUNT];
 
 // ---- kernel launchers (each returns hipGetLastError()) ----

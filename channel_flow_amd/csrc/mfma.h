@@ -288,26 +288,40 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
   }
 }
 
-template <class P, int RBW>
-hipError_t run_mfma_cos_rbw(hipStream_t st, const P& p, int nz) {
-  // 8 accumulator blocks per wave keeps 2-3 waves per SIMD to hide LDS/HBM latency
-  constexpr int budget = 8 / (P::NACC * RBW) > 0 ? 8 / (P::NACC * RBW) : 1;
-  constexpr int CB = budget >= 4 ? 4 : (budget >= 2 ? 2 : 1);
+template <class P, int RBW, int CB>
+hipError_t launch_mfma_cos(hipStream_t st, const P& p, int nz) {
   const long tiles = (p.ncold + 16 * CB - 1) / (16 * CB);
   const size_t shm = cos_recurrence<P>() ? 0 : sizeof(typename P::TC) * p.n4;
   hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB>), dim3((unsigned)tiles, 1, nz), dim3(256), shm, st, p);
   return hipGetLastError();
 }
 
+// cb = column blocks (16 columns each) per wave, as picked by the autotuner;
+// 0 or an unsupported value takes the default (8 accumulator blocks per wave,
+// which keeps 2-3 waves per SIMD).  Choices are kept within 16 accumulator
+// blocks per wave.  The tile choice does not change any output's summation
+// order, so every choice gives bit-identical results.
+template <class P, int RBW>
+hipError_t run_mfma_cos_rbw(hipStream_t st, const P& p, int nz, int cb) {
+  constexpr int budget = 8 / (P::NACC * RBW) > 0 ? 8 / (P::NACC * RBW) : 1;
+  constexpr int CBH = budget >= 4 ? 4 : (budget >= 2 ? 2 : 1);
+  if (cb == 1) return launch_mfma_cos<P, RBW, 1>(st, p, nz);
+  if constexpr (P::NACC * RBW * 2 <= 16)
+    if (cb == 2) return launch_mfma_cos<P, RBW, 2>(st, p, nz);
+  if constexpr (P::NACC * RBW * 4 <= 16)
+    if (cb == 4) return launch_mfma_cos<P, RBW, 4>(st, p, nz);
+  return launch_mfma_cos<P, RBW, CBH>(st, p, nz);
+}
+
 template <class P>
-hipError_t run_mfma_cos(hipStream_t st, const P& p, int nz) {
+hipError_t run_mfma_cos(hipStream_t st, const P& p, int nz, int cb = 0) {
   const int rbw = (p.sh.MB + 3) / 4;
   switch (rbw) {
-    case 1: return run_mfma_cos_rbw<P, 1>(st, p, nz);
-    case 2: return run_mfma_cos_rbw<P, 2>(st, p, nz);
-    case 3: return run_mfma_cos_rbw<P, 3>(st, p, nz);
-    case 4: return run_mfma_cos_rbw<P, 4>(st, p, nz);
-    case 5: return run_mfma_cos_rbw<P, 5>(st, p, nz);
+    case 1: return run_mfma_cos_rbw<P, 1>(st, p, nz, cb);
+    case 2: return run_mfma_cos_rbw<P, 2>(st, p, nz, cb);
+    case 3: return run_mfma_cos_rbw<P, 3>(st, p, nz, cb);
+    case 4: return run_mfma_cos_rbw<P, 4>(st, p, nz, cb);
+    case 5: return run_mfma_cos_rbw<P, 5>(st, p, nz, cb);
     default: return hipErrorInvalidValue;
   }
 }

@@ -133,6 +133,14 @@ module chf_binding
       real(c_double), intent(out) :: total_ms
       integer(c_int), intent(out) :: launches
     end function chf_timing
+
+    ! autotuner report: chosen column blocks per wave of kernel class id
+    integer(c_int) function chf_tile_choice(ctx, id, ms_by_choice) bind(C, name="chf_tile_choice")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: id
+      real(c_double), intent(out) :: ms_by_choice(3)
+    end function chf_tile_choice
   end interface
 
 contains

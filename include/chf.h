@@ -112,6 +112,14 @@ int chf_kernel_count(void);
 const char* chf_kernel_name(int id);
 int chf_timing(void* ctx, int id, double* total_ms, int* launches);
 
+/* Autotuner report (no counterpart in the reference): chf_create times the
+ * cos-table contractions (kernel classes synth_padded, analysis, synth_points,
+ * rhs_v, rhs_o) with 1, 2 and 4 column blocks per wavefront on the context's
+ * own shapes and keeps the fastest.  Returns the chosen count (0 for classes
+ * without a tile choice); ms_by_choice (may be NULL) receives the three
+ * measured times per launch.  The choice never changes results. */
+int chf_tile_choice(void* ctx, int id, double* ms_by_choice);
+
 /* Ensemble: n independent channels (any mix of grid sizes, Reynolds numbers
  * and precisions) advanced together, one step of every member being one
  * hipGraph launch in which the members run as parallel branches.  The

@@ -258,3 +258,15 @@ def test_long_streamwise_lines(require_gpu, nnx):
         assert e < TRAJ_TOL, (k, e)
     for k in ("vnp1", "omegaynp1"):
         assert em[k] < 1e-5, (k, em[k])
+
+
+def test_autotuner_reports_choices(require_gpu):
+    """chf_create tunes the column blocks per wavefront of the five cos-table
+    contractions; each has a choice of 1, 2 or 4 and positive timings (the
+    parity tests above run with these tuned choices)."""
+    with make_channel(SMALL) as ch:
+        tiles = ch.tile_choices()
+    assert set(tiles) == {"synth_padded", "analysis", "synth_points", "rhs_v", "rhs_o"}, tiles
+    for name, t in tiles.items():
+        assert t["choice"] in (1, 2, 4), (name, t)
+        assert all(ms > 0 for ms in t["ms"]), (name, t)
