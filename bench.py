@@ -36,6 +36,10 @@ def dims(c):
              Py=2 * N + 1, Ph=N + 1, ne=N // 2 + 1, no=(N + 1) // 2, Pn=N // 2 + 1)
     d["ncol"] = d["Mz"] * d["Mx"]
     d["ngroups"] = d["Mx"] * (d["Mz"] // 2)
+    # operator classes: groups with equal k^2 share one set of inverses
+    al = [i / c["aphi"] for i in range(d["Mx"])]
+    be = [(j if j <= d["Mz"] // 2 else j - d["Mz"]) / c["beta"] for j in range(d["Mz"])]
+    d["nclass"] = len({al[g % d["Mx"]] ** 2 + be[g // d["Mx"]] ** 2 for g in range(d["ngroups"])})
     return d
 
 
@@ -56,7 +60,7 @@ def kernel_work(d, precision="fp64"):
     ne, no, Ny, Ph, Pn = d["ne"], d["no"], d["Ny"], d["Ph"], d["Pn"]
     rv_rows = (ne - 2) + (no - 2)
     ro_rows = (ne - 1) + (no - 1)
-    inv_v_bytes = (d["ngroups"] - 1) * (ne * ne + no * no) * 8   # group 0 has no v system
+    inv_v_bytes = (d["nclass"] - 1) * (ne * ne + no * no) * 8   # one per k^2 class; k^2 = 0 has no v system
     ws_flops = 2.0 * col * ((ne - 1) * (ne - 1) + ne * (ne - 1) + (no - 1) * (no - 1) + no * (no - 1))
     return {
         "synth_padded": (6 * (S + Y), 2.0 * Ph * Ny * col * 6),

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <vector>
 
 #include "chf_internal.h"
@@ -149,7 +150,7 @@ void free_all(Ctx* c) {
   f(c->hvc); f(c->hgc);
   f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
-  f(c->inv); f(c->inv0);
+  f(c->inv); f(c->inv0); f(c->crep); f(c->bmat); f(c->bg0); f(c->bg1);
   for (int i = 0; i < 2; ++i) { f(c->wL[i]); f(c->wR[i]); f(c->wlam[i]); }
   f(c->phys_c); f(c->phys_r[0]); f(c->phys_r[1]); f(c->phys_r[2]); f(c->stats); f(c->twxu); f(c->twzu); f(c->twx); f(c->twz); f(c->costab);
   f(c->gv); f(c->go);
@@ -406,6 +407,45 @@ int build(Ctx* c) {
   CHF_TRY(hipMemcpy(c->alpha, al.data(), d.Mx * sizeof(double), hipMemcpyHostToDevice));
   CHF_TRY(hipMemcpy(c->beta, be.data(), d.Mz * sizeof(double), hipMemcpyHostToDevice));
 
+  // operator classes: the collocation operators depend on k^2 only, so groups
+  // whose k^2 (computed as the set-up and solve kernels compute it) is bitwise
+  // equal share one set of inverses; solve blocks take up to two groups of a
+  // class, in order of their first group
+  {
+    std::map<double, int> cls_of;
+    std::vector<std::vector<int>> members;
+    for (int g = 0; g < d.ngroups; ++g) {
+      const int kx = g % d.Mx, kz = g / d.Mx;
+      const double k2 = al[kx] * al[kx] + be[kz] * be[kz];
+      auto it = cls_of.find(k2);
+      if (it == cls_of.end()) {
+        cls_of.emplace(k2, (int)members.size());
+        members.push_back({g});
+      } else {
+        members[it->second].push_back(g);
+      }
+    }
+    std::vector<int> crep, bmat, bg0, bg1;
+    for (size_t k = 0; k < members.size(); ++k) {
+      crep.push_back(members[k][0]);
+      for (size_t i = 0; i < members[k].size(); i += 2) {
+        bmat.push_back((int)k);
+        bg0.push_back(members[k][i]);
+        bg1.push_back(i + 1 < members[k].size() ? members[k][i + 1] : -1);
+      }
+    }
+    d.nclass = (int)crep.size();
+    d.nblocks = (int)bmat.size();
+    CHF_TRY(dalloc(c, &c->crep, crep.size()));
+    CHF_TRY(dalloc(c, &c->bmat, bmat.size()));
+    CHF_TRY(dalloc(c, &c->bg0, bg0.size()));
+    CHF_TRY(dalloc(c, &c->bg1, bg1.size()));
+    CHF_TRY(hipMemcpy(c->crep, crep.data(), crep.size() * sizeof(int), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->bmat, bmat.data(), bmat.size() * sizeof(int), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->bg0, bg0.data(), bg0.size() * sizeof(int), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->bg1, bg1.data(), bg1.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+
   // SCHEB operator matrices
   std::vector<double> t0, t1, t2, t4;
   scheb(d.N, t0, t1, t2, t4);
@@ -426,7 +466,7 @@ int build(Ctx* c) {
     if (c->wdiag) {
       d.inv_row = neP + noP;
       d.inv_stride = (long)d.ne * d.inv_row;
-      CHF_TRY(dalloc(c, &c->inv, (size_t)d.ngroups * d.inv_stride));
+      CHF_TRY(dalloc(c, &c->inv, (size_t)d.nclass * d.inv_stride));
       CHF_TRY(dalloc(c, &c->inv0, (size_t)d.ne * (neP + noP)));
       c->inv_w = c->inv0;
       c->stride_w = 0;
@@ -434,7 +474,7 @@ int build(Ctx* c) {
     } else {
       d.inv_row = 2 * (neP + noP);
       d.inv_stride = (long)d.ne * d.inv_row;
-      CHF_TRY(dalloc(c, &c->inv, (size_t)d.ngroups * d.inv_stride));
+      CHF_TRY(dalloc(c, &c->inv, (size_t)d.nclass * d.inv_stride));
       c->inv_w = c->inv + (neP + noP);
       c->stride_w = d.inv_stride;
       c->row_w = d.inv_row;

@@ -24,7 +24,9 @@ struct Dims {
   int no;     // odd degrees 1,3,..: (N+1)/2
   int Pn;     // N+1-point grid folded by parity: N/2+1
   long ncol;  // complex columns per plane: Mz*Mx
-  int ngroups;  // (kx,|kz|) operator groups: Mx*(Mz/2)
+  int ngroups;  // (kx,|kz|) groups: Mx*(Mz/2)
+  int nclass;   // operator classes (groups with bitwise-equal k^2 share an inverse)
+  int nblocks;  // solve blocks: up to two groups of one class each
   long inv_row;     // doubles per inverse row j: 2 (neP + noP), the four half systems side by side
   long inv_stride;  // doubles per group: ne * inv_row
 };
@@ -111,6 +113,8 @@ struct Ctx {
   // (inv_w = inv + neP + noP, same strides), or the operator is diagonalised
   // (wdiag) and only the mean group's k^2 = 0 inverse is stored, in inv0.
   bool wdiag = false;
+  int* crep = nullptr;                               // [nclass] first group of each class
+  int *bmat = nullptr, *bg0 = nullptr, *bg1 = nullptr;  // [nblocks] class, groups (-1: none)
   double* inv0 = nullptr;
   const double* inv_w = nullptr;
   long stride_w = 0, row_w = 0;

@@ -28,6 +28,7 @@ struct SetupArgs {
   int Mx, N, Ny, ne, no;
   long stride, row, stride_w, row_w;
   int wdiag;        // omega operator diagonalised: only group 0's omega halves are stored
+  const int* crep;  // representative group of each operator class
   double dt, re;
 };
 
@@ -44,12 +45,15 @@ __device__ inline double op_w(const SetupArgs& a, double k2, int p, int l) {
   return -a.t2[o] * a.dt * 0.5 / a.re + (1 + k2 * a.dt * 0.5 / a.re) * a.t0[o];
 }
 
-// grid: (ngroups, 4) ; blockIdx.y = eq*2 + parity (eq 0 = v, 1 = omega_y)
+// grid: (operator classes, 4) ; blockIdx.y = eq*2 + parity (eq 0 = v, 1 = omega_y).
+// A class is the set of groups whose k^2 is bitwise equal; it is inverted once,
+// at its first group's k^2.
 __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
   extern __shared__ double sm[];
   __shared__ int piv[160];
   __shared__ double colk[160];
-  const int g = blockIdx.x;
+  const int cls = blockIdx.x;
+  const int g = a.crep[cls];
   const int eq = blockIdx.y >> 1, par = blockIdx.y & 1;
   const int kx = g % a.Mx, kz = g / a.Mx;
   const double k2 = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
@@ -58,7 +62,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs a) {
   const int neP = a.ne + (a.ne & 1);
   if (eq == 1 && g > 0 && a.wdiag) return;
   const long orow = eq ? a.row_w : a.row;
-  double* out = (eq ? a.inv_w + (long)g * a.stride_w : a.inv + (long)g * a.stride) + par * neP;
+  double* out = (eq ? a.inv_w + (long)cls * a.stride_w : a.inv + (long)cls * a.stride) + par * neP;
   if (eq == 0 && g == 0) {  // k^2 = 0 has no v system (PF:184, 247)
     for (int idx = threadIdx.x; idx < a.ne * nP; idx += blockDim.x)
       out[(long)(idx / nP) * orow + idx % nP] = 0.0;
@@ -158,6 +162,7 @@ hipError_t launch_setup_inverses(Ctx& c) {
   a.stride_w = c.stride_w;
   a.row_w = c.row_w;
   a.wdiag = c.wdiag ? 1 : 0;
+  a.crep = c.crep;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
   size_t shm = sizeof(double) * (size_t)d.ne * d.ne;
@@ -165,7 +170,7 @@ hipError_t launch_setup_inverses(Ctx& c) {
     hipError_t e = hipFuncSetAttribute((const void*)k_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_setup, dim3(d.ngroups, 4), dim3(256), shm, c.stream, a);
+  hipLaunchKernelGGL(k_setup, dim3(d.nclass, 4), dim3(256), shm, c.stream, a);
   return hipGetLastError();
 }
 
@@ -182,6 +187,7 @@ struct SolveArgs {
   int Mx, Mz, N, Ny, ne, no;
   long stride, row, stride_w, row_w, ncol;
   int wdiag;
+  const int *bmat, *bg0, *bg1;  // per block: operator class, first and second group (-1: none)
   double dt, re;
 };
 
@@ -201,41 +207,49 @@ __device__ __forceinline__ double2 stream_load(const double* p) {
   return make_double2(t.x, t.y);
 }
 
-// One block per (kx, |kz|) group; modes m0 = (kz, kx), m1 = (Mz - kz, kx) share
-// the operator.  The block streams the group's four half-system inverses once,
-// multiplying each by the 4 real right-hand-side vectors (Re, Im of both modes),
-// and writes v^{n+1} and omega_y^{n+1} (PF:242-394).  Group 0 is the mean mode
-// (PF:401-474): U and W with the omega_y-type operator at k^2 = 0.
+// One block per (operator class, up to two groups).  A group (kx, |kz|) holds
+// the modes m0 = (kz, kx) and m1 = (Mz - kz, kx); groups with the same k^2
+// share the operator, so the block streams the class's half-system inverses
+// once, multiplies each by the 4 real right-hand-side vectors (Re, Im of both
+// modes) of each of its groups, and writes v^{n+1} (and omega_y^{n+1} when that
+// operator is not diagonalised) (PF:242-394).  The block holding group 0 is
+// the mean mode (PF:401-474): U and W with the omega_y-type operator at k^2 = 0.
 __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
-  extern __shared__ double rb[];  // [2 eq][4 vec][Ny] right-hand sides
+  extern __shared__ double rb[];  // [slot][4 vec][Ny], slot = group * neq + eq
   const int Ny = a.Ny, N = a.N, ne = a.ne, no = a.no;
-  const int g = xcd_group(blockIdx.x, gridDim.x);
-  const int kx = g % a.Mx, kzab = g / a.Mx;
-  const bool mean = (g == 0);
-  const int kz0 = kzab, kz1 = kzab ? a.Mz - kzab : 0;
-  const long col0 = (long)kz0 * a.Mx + kx, col1 = (long)kz1 * a.Mx + kx;
+  const int b = xcd_group(blockIdx.x, gridDim.x);
+  const int mat = a.bmat[b];
+  const int gs0 = a.bg0[b], gs1 = a.bg1[b];
+  const int ngr = gs1 >= 0 ? 2 : 1;
+  const bool mean = (gs0 == 0);
+  const int neq = a.wdiag ? 1 : 2;  // equations this kernel solves for a non-mean group
   const double c = a.dt * 0.5 / a.re;
 
   // ---- right-hand sides, parity-folded (rows of the half systems)
   if (!mean) {
-    for (int idx = threadIdx.x; idx < 2 * Ny; idx += blockDim.x) {
-      const int eq = idx / Ny, row = idx - eq * Ny;
-      // row < ne: even system row; else odd system row (row - ne)
-      const bool odd = row >= ne;
-      const int i = odd ? row - ne : row;
-      const int nbc = eq ? 1 : 2;
-      double2 x0 = make_double2(0, 0), x1 = make_double2(0, 0);
-      if (i >= nbc) {
-        const int r = i - nbc;
-        const double2* src = eq ? (odd ? a.roO : a.roE) : (odd ? a.rvO : a.rvE);
-        x0 = src[(long)r * a.ncol + col0];
-        if (kzab > 0) x1 = src[(long)r * a.ncol + col1];
+    for (int gi = 0; gi < ngr; ++gi) {
+      const int g = gi ? gs1 : gs0;
+      const int kx = g % a.Mx, kzab = g / a.Mx;
+      const long col0 = (long)kzab * a.Mx + kx, col1 = (long)(kzab ? a.Mz - kzab : 0) * a.Mx + kx;
+      for (int idx = threadIdx.x; idx < neq * Ny; idx += blockDim.x) {
+        const int eq = idx / Ny, row = idx - eq * Ny;
+        // row < ne: even system row; else odd system row (row - ne)
+        const bool odd = row >= ne;
+        const int i = odd ? row - ne : row;
+        const int nbc = eq ? 1 : 2;
+        double2 x0 = make_double2(0, 0), x1 = make_double2(0, 0);
+        if (i >= nbc) {
+          const int r = i - nbc;
+          const double2* src = eq ? (odd ? a.roO : a.roE) : (odd ? a.rvO : a.rvE);
+          x0 = src[(long)r * a.ncol + col0];
+          if (kzab > 0) x1 = src[(long)r * a.ncol + col1];
+        }
+        double* dst = rb + (gi * neq + eq) * 4 * Ny;
+        dst[0 * Ny + row] = x0.x;
+        dst[1 * Ny + row] = x0.y;
+        dst[2 * Ny + row] = x1.x;
+        dst[3 * Ny + row] = x1.y;
       }
-      double* dst = rb + eq * 4 * Ny;
-      dst[0 * Ny + row] = x0.x;
-      dst[1 * Ny + row] = x0.y;
-      dst[2 * Ny + row] = x1.x;
-      dst[3 * Ny + row] = x1.y;
     }
   } else {
     // mean mode: U (vectors 0,1) and W (vectors 2,3) with the omega_y-type
@@ -266,27 +280,28 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
           acc[2] += 0.5 * (Gp.x + s * Gq.x); acc[3] += 0.5 * (Gp.y + s * Gq.y);
         }
       }
-      double* dst = rb + 4 * Ny;
-      for (int q4 = 0; q4 < 4; ++q4) dst[q4 * Ny + row] = acc[q4];
+      for (int q4 = 0; q4 < 4; ++q4) rb[q4 * Ny + row] = acc[q4];  // slot 0
     }
   }
   __syncthreads();
-  // ---- dense half-system inverses: x = Einv be, Oinv bo.  Thread = one row
-  // pair (2i, 2i+1) of one of the four half systems; 16-byte matrix loads.
+  // ---- dense half-system inverses x = Einv be, Oinv bo.  Thread = one row pair
+  // (2i, 2i+1) of one half system (of one equation), 16-byte matrix loads,
+  // accumulating the vectors of every group of the block.
   const int neP = ne + (ne & 1), noP = no + (no & 1);
-  const int pe = neP / 2, po = noP / 2;           // row pairs per half system
-  for (int t = threadIdx.x; t < 2 * (pe + po); t += blockDim.x) {
-    const int eq = t / (pe + po);
-    if (mean && eq == 0) continue;
-    if (!mean && eq == 1 && a.wdiag) continue;  // omega_y by the diagonalised solve
-    const int rr = t - eq * (pe + po);
+  const int pe = neP / 2, po = noP / 2;  // row pairs per half system
+  const int ntask = (mean ? 1 : neq) * (pe + po);
+  for (int t = threadIdx.x; t < ntask; t += blockDim.x) {
+    const int eq = mean ? 1 : t / (pe + po);
+    const int rr = mean ? t : t - eq * (pe + po);
     const bool odd = rr >= pe;
     const int i2 = 2 * (odd ? rr - pe : rr);
     const int n = odd ? no : ne;
     const long nP = eq ? a.row_w : a.row;
-    const double* M = (eq ? a.inv_w + (long)g * a.stride_w : a.inv + (long)g * a.stride) + (odd ? neP : 0) + i2;
-    const double* rhs = rb + eq * 4 * Ny + (odd ? ne : 0);
-    double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double* M = (eq ? a.inv_w + (long)mat * a.stride_w : a.inv + (long)mat * a.stride) + (odd ? neP : 0) + i2;
+    const int ns = mean ? 1 : ngr;  // vector sets (slots) for this equation
+    const double* rhs0 = rb + (mean ? 0 : eq) * 4 * Ny + (odd ? ne : 0);
+    const double* rhs1 = rb + (neq + eq) * 4 * Ny + (odd ? ne : 0);
+    double x[2][8] = {};
     int j = 0;
     for (; j + 4 <= n; j += 4) {
       double2 m4[4];
@@ -296,9 +311,17 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
       for (int u = 0; u < 4; ++u) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double r = rhs[q * Ny + j + u];
-          x[q] = fma(m4[u].x, r, x[q]);
-          x[4 + q] = fma(m4[u].y, r, x[4 + q]);
+          const double r = rhs0[q * Ny + j + u];
+          x[0][q] = fma(m4[u].x, r, x[0][q]);
+          x[0][4 + q] = fma(m4[u].y, r, x[0][4 + q]);
+        }
+        if (ns > 1) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double r = rhs1[q * Ny + j + u];
+            x[1][q] = fma(m4[u].x, r, x[1][q]);
+            x[1][4 + q] = fma(m4[u].y, r, x[1][4 + q]);
+          }
         }
       }
     }
@@ -306,23 +329,34 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
       const double2 m = stream_load(M + (long)j * nP);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double r = rhs[q * Ny + j];
-        x[q] = fma(m.x, r, x[q]);
-        x[4 + q] = fma(m.y, r, x[4 + q]);
+        const double r = rhs0[q * Ny + j];
+        x[0][q] = fma(m.x, r, x[0][q]);
+        x[0][4 + q] = fma(m.y, r, x[0][4 + q]);
+        if (ns > 1) {
+          const double r1 = rhs1[q * Ny + j];
+          x[1][q] = fma(m.x, r1, x[1][q]);
+          x[1][4 + q] = fma(m.y, r1, x[1][4 + q]);
+        }
       }
     }
+    for (int gi = 0; gi < ns; ++gi) {
+      const int g = gi ? gs1 : gs0;
+      const int kx = g % a.Mx, kzab = g / a.Mx;
+      const long col0 = (long)kzab * a.Mx + kx, col1 = (long)(kzab ? a.Mz - kzab : 0) * a.Mx + kx;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int i = i2 + h;
-      if (i >= n) break;
-      const int l = 2 * i + (odd ? 1 : 0);
-      if (mean) {
-        a.au0[l] = make_double2(x[4 * h + 0], x[4 * h + 1]);
-        a.w0[l] = make_double2(x[4 * h + 2], x[4 * h + 3]);
-      } else {
-        double2* dst = eq ? a.oy : a.v;
-        dst[(long)l * a.ncol + col0] = make_double2(x[4 * h + 0], x[4 * h + 1]);
-        if (kzab > 0) dst[(long)l * a.ncol + col1] = make_double2(x[4 * h + 2], x[4 * h + 3]);
+      for (int h = 0; h < 2; ++h) {
+        const int i = i2 + h;
+        if (i >= n) break;
+        const int l = 2 * i + (odd ? 1 : 0);
+        const double* xx = x[gi];
+        if (mean) {
+          a.au0[l] = make_double2(xx[4 * h + 0], xx[4 * h + 1]);
+          a.w0[l] = make_double2(xx[4 * h + 2], xx[4 * h + 3]);
+        } else {
+          double2* dst = eq ? a.oy : a.v;
+          dst[(long)l * a.ncol + col0] = make_double2(xx[4 * h + 0], xx[4 * h + 1]);
+          if (kzab > 0) dst[(long)l * a.ncol + col1] = make_double2(xx[4 * h + 2], xx[4 * h + 3]);
+        }
       }
     }
   }
@@ -472,17 +506,21 @@ hipError_t launch_solve(Ctx& c) {
   a.stride_w = c.stride_w;
   a.row_w = c.row_w;
   a.wdiag = c.wdiag ? 1 : 0;
+  a.bmat = c.bmat;
+  a.bg0 = c.bg0;
+  a.bg1 = c.bg1;
   a.ncol = d.ncol;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
-  const size_t shm = sizeof(double) * 8ull * d.Ny;
+  // two groups per block, one or two equations each
+  const size_t shm = sizeof(double) * 4ull * d.Ny * 2 * (c.wdiag ? 1 : 2);
   if (shm > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
   // with the omega_y operator diagonalised a block streams the two v half
   // systems only: ne/2 + no/2 row pairs, so 128 threads
-  hipLaunchKernelGGL(k_solve, dim3(d.ngroups), dim3(c.wdiag ? 128 : 256), shm, c.stream, a);
+  hipLaunchKernelGGL(k_solve, dim3(d.nblocks), dim3(c.wdiag ? 128 : 256), shm, c.stream, a);
   return hipGetLastError();
 }
 
