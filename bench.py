@@ -102,7 +102,7 @@ def roofline_of(name, avg_ms, work, precision="fp64"):
 
 # HIP kernel symbol behind each timed kernel class (the FP32 forms share symbols
 # with the FP64 ones; template arguments differ).
-KERNEL_SYMBOL = {"synth_padded": "k_mfma_cos", "zfft_inv": "k_zfft", "xpass": "k_xpass",
+KERNEL_SYMBOL = {"synth_padded": "k_mfma_cos", "zfft_inv": "k_zfft", "xpass": "k_xpass_st",
                  "zfft_fwd": "k_zfft", "analysis": "k_mfma_cos", "hassemble": "k_hassemble",
                  "synth_points": "k_mfma_cos", "mean_points": "k_mean_points",
                  "rhs_prep": "k_rhs_prep", "rhs_v": "k_mfma_cos", "rhs_o": "k_mfma_cos",
