@@ -79,7 +79,7 @@ def kernel_work(d, precision="fp64"):
         # v solve (omega_y of the mean group only) and the diagonalised omega_y solve
         "solve": (inv_v_bytes + rv_rows / Ny * S + S, 0.0),
         "wsolve": (ro_rows / Ny * S + S, ws_flops),
-        "kinematics": (6 * S, 0.0),
+        "kinematics": (8 * S, 0.0),   # reads v, omega_y; writes u, w, omega_x, omega_z and the two brackets
     }
 
 

@@ -539,7 +539,7 @@ int build(Ctx* c) {
   }
 
   // zero the state
-  for (double2* q : {c->v, c->oy, c->u, c->w, c->ox, c->oz, c->hv[0], c->hv[1], c->hg[0], c->hg[1]})
+  for (double2* q : {c->v, c->oy, c->u, c->w, c->ox, c->oz, c->hv[0], c->hv[1], c->hg[0], c->hg[1], c->gv, c->go})
     CHF_TRY(hipMemsetAsync(q, 0, fld * sizeof(double2), c->stream));
   for (double2* q : {c->cl1[0], c->cl1[1], c->cl3[0], c->cl3[1], c->au0, c->w0})
     CHF_TRY(hipMemsetAsync(q, 0, d.Ny * sizeof(double2), c->stream));
@@ -607,7 +607,10 @@ int nonlinear(Ctx* c) {
 }
 
 int solve(Ctx* c) {
-  CHF_TRY(timed(c, K_RHS_PREP, [&] { return launch_rhs_prep(*c); }));
+  if (c->prep_stale) {
+    CHF_TRY(timed(c, K_RHS_PREP, [&] { return launch_rhs_prep(*c); }));
+    c->prep_stale = false;
+  }
   CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
   CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
   CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c); }));
@@ -672,6 +675,7 @@ int chf_set_state(void* ctx, const chf_state* st) {
     if (!e.src) return fail(c, "chf_set_state: null array");
   for (auto& e : m) CHF_TRY(hipMemcpyAsync(e.dst, e.src, e.bytes, hipMemcpyHostToDevice, c->stream));
   CHF_TRY(hipStreamSynchronize(c->stream));
+  c->prep_stale = true;  // the RHS brackets of the old v, omega_y no longer apply
   return 0;
 }
 
@@ -712,8 +716,9 @@ int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1, cons
   for (double2* q : {c->cl1[0], c->cl1[1], c->cl3[0], c->cl3[1]})
     CHF_TRY(hipMemsetAsync(q, 0, mb, c->stream));
   // fields with all-zero u, w, omega_x, omega_z at the Nyquist column
-  for (double2* q : {c->u, c->w, c->ox, c->oz}) CHF_TRY(hipMemsetAsync(q, 0, fb, c->stream));
+  for (double2* q : {c->u, c->w, c->ox, c->oz, c->gv, c->go}) CHF_TRY(hipMemsetAsync(q, 0, fb, c->stream));
   CHF_TRY(launch_kinematics(*c));
+  c->prep_stale = false;  // the kinematics pass wrote the brackets
   int rc = nonlinear(c);
   if (rc) return rc;
   CHF_TRY(hipStreamSynchronize(c->stream));
@@ -1065,6 +1070,10 @@ int chf_ensemble_step(void* ens, int n) {
   if (n < 0) return ens_fail(e, "chf_ensemble_step: negative step count");
   for (Ctx* c : e->m) {
     if (c->timing) return ens_fail(e, "chf_ensemble_step: turn member timing off first");
+    if (c->prep_stale) {  // brackets after a state change: computed once, outside the graph
+      ENS_TRY(launch_rhs_prep(*c));
+      c->prep_stale = false;
+    }
     // member work queued outside the graph (set_state, init) must finish first
     ENS_TRY(hipStreamSynchronize(c->stream));
   }

@@ -96,6 +96,9 @@ struct Ctx {
   // solve work
   double2 *rvE = nullptr, *rvO = nullptr, *roE = nullptr, *roO = nullptr;
   double2 *gv = nullptr, *go = nullptr;  // explicit-operator coefficient vectors
+  // gv/go are written by the kinematics pass for the next solve; after the
+  // state is replaced (chf_set_state) a standalone pass recomputes them
+  bool prep_stale = true;
   // operators
   double* host_mats = nullptr;
   double *tsyn = nullptr, *tana = nullptr, *tpts = nullptr, *t0 = nullptr, *t2 = nullptr,
@@ -182,7 +185,7 @@ hipError_t launch_rhs_prep(Ctx& c);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
 hipError_t launch_solve(Ctx& c);       // v, omega_y and the mean mode
 hipError_t launch_wsolve(Ctx& c);      // diagonalised omega_y solve (all modes but the mean)
-hipError_t launch_kinematics(Ctx& c);  // u, w, omega_x, omega_z from v, omega_y, U, W
+hipError_t launch_kinematics(Ctx& c);  // u, w, omega_x, omega_z (and the next RHS brackets) from v, omega_y, U, W
 hipError_t launch_setup_inverses(Ctx& c);
 
 }  // namespace chf

@@ -376,12 +376,20 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
 // Dv, u, w at parity 1-q and Du, Dw, omega_x, omega_z at parity q.  One thread
 // runs one chain of one mode, backwards over the degree, with its loads issued
 // eight degrees ahead.
+//
+// The same pass forms the explicit-operator brackets of the next solve's
+// right-hand sides (get_coef_rhs / get_rhs_omegay, PF:252-257, 335-340):
+//   g_v = c D^4 v + (1 - 2 c k^2) D^2 v - k^2 (1 - c k^2) v   (at parity q)
+//   g_w = c D^2 omega_y + (1 - c k^2) omega_y                  (at parity 1-q)
+// whose chains read the same v and omega_y degrees as chain q.
 struct KinArgs {
   const double2 *v, *oy, *au0, *w0;
   double2 *u, *w, *ox, *oz;
+  double2 *gv, *go;  // brackets for the next solve
   const double *al, *be;
   int Mx, Mz, N, Ny;
   long ncol;
+  double c;          // dt / (2 re)
 };
 
 __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
@@ -431,6 +439,10 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
   double2 dv2 = make_double2(0, 0);                   // Dv_{k+2} (unhalved)
   double2 up = make_double2(0, 0), wp = make_double2(0, 0);    // u_{k+1}, w_{k+1}
   double2 du2 = make_double2(0, 0), dw2 = make_double2(0, 0);  // Du_{k+2}, Dw_{k+2}
+  // bracket chains: latest unhalved D^2 v, D^3 v, D^4 v, D omega, D^2 omega and omega_{k+1}
+  double2 d2s = make_double2(0, 0), d3s = make_double2(0, 0), d4s = make_double2(0, 0);
+  double2 e1s = make_double2(0, 0), e2s = make_double2(0, 0), wn = make_double2(0, 0);
+  const double cc = a.c, c2 = 1.0 - 2.0 * cc * k2, c0 = k2 * (1.0 - cc * k2), cw = 1.0 - cc * k2;
   for (int k0 = N; k0 >= 0; k0 -= 8) {
     double2 in[8];
 #pragma unroll
@@ -456,6 +468,16 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
         const double2 vk = in[s];
         a.ox[off] = make_double2(dw.x + be * vk.y, dw.y - be * vk.x);
         a.oz[off] = make_double2(-du.x - al * vk.y, -du.y + al * vk.x);
+        // brackets: D^2 v, D^4 v at k from D v, D^3 v at k+1; D omega at k from omega_{k+1}
+        double2 q2 = make_double2(d2s.x + f * dv2.x, d2s.y + f * dv2.y);
+        double2 q4 = make_double2(d4s.x + f * d3s.x, d4s.y + f * d3s.y);
+        e1s = make_double2(e1s.x + f * wn.x, e1s.y + f * wn.y);
+        d2s = q2;
+        d4s = q4;
+        if (k == 0) {
+          q2.x *= 0.5; q2.y *= 0.5; q4.x *= 0.5; q4.y *= 0.5;
+        }
+        a.gv[off] = make_double2(cc * q4.x + c2 * q2.x - c0 * vk.x, cc * q4.y + c2 * q2.y - c0 * vk.y);
         vp = vk;
       } else {
         // Dv_k from v_{k+1}; u_k, w_k with omega_y_k
@@ -471,6 +493,15 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
         a.w[off] = wk;
         up = uk;
         wp = wk;
+        // brackets: D^3 v at k from D^2 v at k+1; D^2 omega at k from D omega at k+1
+        d3s = make_double2(d3s.x + f * d2s.x, d3s.y + f * d2s.y);
+        double2 r2 = make_double2(e2s.x + f * e1s.x, e2s.y + f * e1s.y);
+        e2s = r2;
+        if (k == 0) {
+          r2.x *= 0.5; r2.y *= 0.5;
+        }
+        a.go[off] = make_double2(cc * r2.x + cw * omr, cc * r2.y + cw * omi);
+        wn = in[s];
       }
     }
   }
@@ -692,6 +723,8 @@ hipError_t launch_kinematics(Ctx& c) {
   a.w = c.w;
   a.ox = c.ox;
   a.oz = c.oz;
+  a.gv = c.gv;
+  a.go = c.go;
   a.al = c.alpha;
   a.be = c.beta;
   a.Mx = d.Mx;
@@ -699,6 +732,7 @@ hipError_t launch_kinematics(Ctx& c) {
   a.N = d.N;
   a.Ny = d.Ny;
   a.ncol = d.ncol;
+  a.c = c.prm.dt * 0.5 / c.prm.re;
   const unsigned nb = (unsigned)((d.ncol + 255) / 256) + 1;  // + the mean-mode block
   const size_t shm = sizeof(double) * 8ull * d.Ny;
   if (shm > 65536) {
