@@ -68,9 +68,8 @@ def kernel_work(d, precision="fp64"):
         "xpass": (9 * Z, 0.0),
         "zfft_fwd": (3 * (Z + Y), 0.0),
         "analysis": (3 * (Y + S), 2.0 * Ny * Ph * col * 3),
-        "hassemble": (5 * S, 0.0),
+        "hassemble": (5 * S + 4 * Ny * 16, 0.0),   # plus the mean-mode point block
         "synth_points": (4 * S, 2.0 * Pn * Ny * col * 2),
-        "mean_points": (4 * Ny * 16, 0.0),
         "rhs_prep": (4 * S, 0.0),
         "rhs_v": (S + 2 * (Ny - 4) / Ny * S + rv_rows / Ny * S,
                   2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
@@ -104,7 +103,7 @@ def roofline_of(name, avg_ms, work, precision="fp64"):
 # with the FP64 ones; template arguments differ).
 KERNEL_SYMBOL = {"synth_padded": "k_mfma_cos", "zfft_inv": "k_zfft", "xpass": "k_xpass_st",
                  "zfft_fwd": "k_zfft", "analysis": "k_mfma_cos", "hassemble": "k_hassemble",
-                 "synth_points": "k_mfma_cos", "mean_points": "k_mean_points",
+                 "synth_points": "k_mfma_cos",
                  "rhs_prep": "k_rhs_prep", "rhs_v": "k_mfma_cos", "rhs_o": "k_mfma_cos",
                  "solve": "k_solve", "kinematics": "k_kinematics", "wsolve": "k_wsolve"}
 

@@ -601,7 +601,6 @@ int nonlinear(Ctx* c) {
   const double2* hc[2] = {c->hvc, c->hgc};
   double2* ho[2] = {c->hv[nxt], c->hg[nxt]};
   CHF_TRY(timed(c, K_SYNTH_POINTS, [&] { return launch_synth_points(*c, hc, ho, 2); }));
-  CHF_TRY(timed(c, K_MEAN_POINTS, [&] { return launch_mean_points(*c); }));
   c->hcur = nxt;
   return 0;
 }

@@ -355,6 +355,46 @@ hipError_t launch_rhs(Ctx& c, int which) {
 
 // ---------------------------------------------------------------- h assembly
 
+struct MArgs {
+  const double2 *lx, *lz;
+  double2 *o1, *o3;
+  const double* tpts;  // [Pn][Ny]
+  int N, Ny, Pn;
+  long ncol;
+};
+
+__device__ void mean_points_block(const MArgs& a) {
+  __shared__ double2 s1[260], s3[260];
+  for (int l = threadIdx.x; l <= a.N; l += blockDim.x) {
+    s1[l] = a.lx[(long)l * a.ncol];
+    s3[l] = a.lz[(long)l * a.ncol];
+  }
+  __syncthreads();
+  for (int jj = threadIdx.x; jj < a.Pn; jj += blockDim.x) {
+    double2 e1 = make_double2(0, 0), o1 = make_double2(0, 0);
+    double2 e3 = make_double2(0, 0), o3 = make_double2(0, 0);
+    for (int l = 0; l <= a.N; ++l) {
+      const double t = a.tpts[jj * a.Ny + l];
+      const double2 x1 = s1[l], x3 = s3[l];
+      if (l & 1) {
+        o1.x += t * x1.x; o1.y += t * x1.y;
+        o3.x += t * x3.x; o3.y += t * x3.y;
+      } else {
+        e1.x += t * x1.x; e1.y += t * x1.y;
+        e3.x += t * x3.x; e3.y += t * x3.y;
+      }
+    }
+    a.o1[jj] = make_double2(e1.x + o1.x, e1.y + o1.y);
+    a.o3[jj] = make_double2(e3.x + o3.x, e3.y + o3.y);
+    const int q = a.N - jj;
+    if (q != jj) {
+      a.o1[q] = make_double2(e1.x - o1.x, e1.y - o1.y);
+      a.o3[q] = make_double2(e3.x - o3.x, e3.y - o3.y);
+    }
+  }
+}
+
+
 struct HArgs {
   const double2 *lx, *ly, *lz;
   double2 *hv, *hg;
@@ -366,7 +406,13 @@ struct HArgs {
 // One thread per complex column; the derivative runs the backward recurrence
 // c_k b_k = b_{k+2} + 2 (k+1) a_{k+1}, which equals the reference's explicit
 // sum b_k = (2/c_k) sum_{j>k, j+k odd} j a_j.
-__global__ __launch_bounds__(256) void k_hassemble(HArgs a) {
+// The launch carries one extra block that computes the mean-mode Lamb point
+// values (mean_points_block) from the same coefficients.
+__global__ __launch_bounds__(256) void k_hassemble(HArgs a, MArgs m) {
+  if (blockIdx.x == gridDim.x - 1) {
+    mean_points_block(m);
+    return;
+  }
   const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cm >= a.ncol) return;
   const int kz = (int)(cm / a.Mx);
@@ -443,64 +489,21 @@ hipError_t launch_hassemble(Ctx& c) {
   a.Mx = c.d.Mx;
   a.N = c.d.N;
   a.ncol = c.d.ncol;
-  hipLaunchKernelGGL(k_hassemble, dim3((unsigned)((a.ncol + 255) / 256)), dim3(256), 0, c.stream, a);
+  MArgs m{};
+  m.lx = c.Lc[0];
+  m.lz = c.Lc[2];
+  m.o1 = c.cl1[1 - c.hcur];
+  m.o3 = c.cl3[1 - c.hcur];
+  m.tpts = c.tpts;
+  m.N = c.d.N;
+  m.Ny = c.d.Ny;
+  m.Pn = c.d.Pn;
+  m.ncol = c.d.ncol;
+  hipLaunchKernelGGL(k_hassemble, dim3((unsigned)((a.ncol + 255) / 256) + 1), dim3(256), 0, c.stream, a, m);
   return hipGetLastError();
 }
 
 // Mean-mode Lamb (column (0,0) of L_x and L_z) -> values at the N+1 points
 // (PF:1592-1609), written into the "new" history slot.
-struct MArgs {
-  const double2 *lx, *lz;
-  double2 *o1, *o3;
-  const double* tpts;  // [Pn][Ny]
-  int N, Ny, Pn;
-  long ncol;
-};
-
-__global__ __launch_bounds__(256) void k_mean_points(MArgs a) {
-  __shared__ double2 s1[260], s3[260];
-  for (int l = threadIdx.x; l <= a.N; l += blockDim.x) {
-    s1[l] = a.lx[(long)l * a.ncol];
-    s3[l] = a.lz[(long)l * a.ncol];
-  }
-  __syncthreads();
-  for (int jj = threadIdx.x; jj < a.Pn; jj += blockDim.x) {
-    double2 e1 = make_double2(0, 0), o1 = make_double2(0, 0);
-    double2 e3 = make_double2(0, 0), o3 = make_double2(0, 0);
-    for (int l = 0; l <= a.N; ++l) {
-      const double t = a.tpts[jj * a.Ny + l];
-      const double2 x1 = s1[l], x3 = s3[l];
-      if (l & 1) {
-        o1.x += t * x1.x; o1.y += t * x1.y;
-        o3.x += t * x3.x; o3.y += t * x3.y;
-      } else {
-        e1.x += t * x1.x; e1.y += t * x1.y;
-        e3.x += t * x3.x; e3.y += t * x3.y;
-      }
-    }
-    a.o1[jj] = make_double2(e1.x + o1.x, e1.y + o1.y);
-    a.o3[jj] = make_double2(e3.x + o3.x, e3.y + o3.y);
-    const int q = a.N - jj;
-    if (q != jj) {
-      a.o1[q] = make_double2(e1.x - o1.x, e1.y - o1.y);
-      a.o3[q] = make_double2(e3.x - o3.x, e3.y - o3.y);
-    }
-  }
-}
-
-hipError_t launch_mean_points(Ctx& c) {
-  MArgs a{};
-  a.lx = c.Lc[0];
-  a.lz = c.Lc[2];
-  a.o1 = c.cl1[1 - c.hcur];
-  a.o3 = c.cl3[1 - c.hcur];
-  a.tpts = c.tpts;
-  a.N = c.d.N;
-  a.Ny = c.d.Ny;
-  a.Pn = c.d.Pn;
-  a.ncol = c.d.ncol;
-  hipLaunchKernelGGL(k_mean_points, dim3(1), dim3(256), 0, c.stream, a);
-  return hipGetLastError();
-}
 
 }  // namespace chf

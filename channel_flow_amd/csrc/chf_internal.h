@@ -180,7 +180,6 @@ hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);
 hipError_t launch_stream_c2r_unpadded(Ctx& c, const double2* in, double* out);
 hipError_t launch_physical(Ctx& c, const double2* field, double* out);
 hipError_t launch_plane_stats(Ctx& c, const double* u, const double* v, const double* w, double* out);
-hipError_t launch_mean_points(Ctx& c);
 hipError_t launch_rhs_prep(Ctx& c);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
 hipError_t launch_solve(Ctx& c);       // v, omega_y and the mean mode

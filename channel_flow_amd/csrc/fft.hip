@@ -542,18 +542,33 @@ __device__ __forceinline__ void st_stage(const V* in, V* out, const V* tw, int j
   for (int r = 0; r < R; ++r) out[spad(base + r * Ns)] = v[r];
 }
 
+// Orders the LDS accesses of one wavefront: its lanes see each other's earlier
+// writes after this point (a wavefront's LDS instructions execute in order;
+// the fences keep the compiler from moving accesses across it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Streamwise pass, one 64-lane wavefront per (y point, span point) line, for
-// L = RA RB RC with L / RC = 64.  The inverse transforms run the stages
-// RA, RB, RC; the last one leaves lane j holding the physical points j + 64 r
-// (r < RC) of every field, so the six products form the Lamb vector in
-// registers, and the forward transforms start with a radix-RC stage on exactly
-// those points (Ns = 1, no twiddle) before RB and RA.  Fields are packed two
-// per complex transform as in k_xpass_reg; block = NL lines.
+// L = RA RB RC with L / RC = 64 and Mx = L / 3 spectral points per row.  Each
+// wave first reads its six input rows into registers (all loads in flight at
+// once), then runs the inverse transforms through its own LDS buffers with
+// wave-level ordering only, so the waves of a block never wait for each other.
+// The inverse transforms run the stages RA, RB, RC; the last one leaves lane j
+// holding the physical points j + 64 r (r < RC) of every field, so the six
+// products form the Lamb vector in registers, and the forward transforms start
+// with a radix-RC stage on exactly those points (Ns = 1, no twiddle) before RB
+// and RA.  Fields are packed two per complex transform as in k_xpass_reg;
+// block = NL lines.
 template <int L, int RA, int RB, int RC, int NL, class V>
 __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
-  static_assert(L == RA * RB * RC && L / RC == 64, "one wavefront per line");
+  static_assert(L == RA * RB * RC && L / RC == 64 && L % 3 == 0, "one wavefront per line");
   using T = decltype(V{}.x);
   constexpr int PL = L + (L >> 4) + 1;  // padded buffer length
+  constexpr int MX = L / 3;             // spectral points per row
+  constexpr int MP = (MX + 63) / 64;    // of them per lane
   extern __shared__ __align__(16) unsigned char smraw[];
   V* sm = reinterpret_cast<V*>(smraw);
   V* tw = sm;
@@ -562,18 +577,28 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
   V* b1 = b0 + PL;
   const long line = (long)blockIdx.x * NL + wave;
   const bool valid = line < a.nlines;
-  const int Mx = a.Mx;
+  V in[6][MP];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+#pragma unroll
+    for (int u = 0; u < MP; ++u) {
+      const int m = lane + 64 * u;
+      in[f][u] = (valid && m < MX) ? a.in[f][line * MX + m] : C2<T>::make(0, 0);
+    }
+  }
   for (int i = threadIdx.x; i < L; i += 64 * NL) tw[i] = a.tw[i];
+  __syncthreads();  // the only block-wide barrier: twiddles
+  if (!valid) return;
   V r[3][RC];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    if (valid) {
-      // natural-order spectrum: F1 + i F2 at m, conj(F1) + i conj(F2) at L - m,
-      // zero on the padding band Mx <= m <= L - Mx; Re of the kx = 0 entry (PF:1116-1118)
-      const V* f1 = a.in[2 * q] + line * Mx;
-      const V* f2 = a.in[2 * q + 1] + line * Mx;
-      for (int m = lane; m < Mx; m += 64) {
-        const V A = f1[m], B = f2[m];
+    // natural-order spectrum: F1 + i F2 at m, conj(F1) + i conj(F2) at L - m,
+    // zero on the padding band Mx <= m <= L - Mx; Re of the kx = 0 entry (PF:1116-1118)
+#pragma unroll
+    for (int u = 0; u < MP; ++u) {
+      const int m = lane + 64 * u;
+      if (m < MX) {
+        const V A = in[2 * q][u], B = in[2 * q + 1][u];
         if (m == 0) {
           b0[spad(0)] = C2<T>::make(A.x, B.x);
         } else {
@@ -581,13 +606,13 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
           b0[spad(L - m)] = C2<T>::make(A.x + B.y, B.x - A.y);
         }
       }
-      for (int m = Mx + lane; m <= L - Mx; m += 64) b0[spad(m)] = C2<T>::make(0, 0);
     }
-    __syncthreads();
+    for (int m = MX + lane; m <= L - MX; m += 64) b0[spad(m)] = C2<T>::make(0, 0);
+    wave_sync();
     st_stage<L, RA, 1, +1>(b0, b1, tw, lane);
-    __syncthreads();
+    wave_sync();
     st_stage<L, RB, RA, +1>(b1, b0, tw, lane);
-    __syncthreads();
+    wave_sync();
     {
       // last inverse stage (Ns = 64): lane j reads j + 64 r, keeps the output in registers
 #pragma unroll
@@ -600,7 +625,7 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
       }
       Dft<RC, +1, V>::run(r[q]);
     }
-    __syncthreads();
+    wave_sync();
   }
   // Lamb vector at x = lane + 64 r: (L_x + i L_y) into r[0], (L_z + 0i) into r[1]
   const T rep = (T)a.re_p;
@@ -622,33 +647,40 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
     Dft<RC, -1, V>::run(r[q]);
 #pragma unroll
     for (int rr = 0; rr < RC; ++rr) b0[spad(lane * RC + rr)] = r[q][rr];
-    __syncthreads();
+    wave_sync();
     st_stage<L, RB, RC, -1>(b0, b1, tw, lane);
-    __syncthreads();
+    wave_sync();
     st_stage<L, RA, RC * RB, -1>(b1, b0, tw, lane);
-    __syncthreads();
-    if (valid) {
-      if (q == 0) {
-        // separate the packed pair: F1 = (G(m) + conj G(-m))/2, F2 = (G(m) - conj G(-m))/(2i)
-        for (int m = lane; m < Mx; m += 64) {
+    wave_sync();
+    if (q == 0) {
+      // separate the packed pair: F1 = (G(m) + conj G(-m))/2, F2 = (G(m) - conj G(-m))/(2i)
+#pragma unroll
+      for (int u = 0; u < MP; ++u) {
+        const int m = lane + 64 * u;
+        if (m < MX) {
           const V Gp = b0[spad(m)];
           const V Gc = b0[spad(m == 0 ? 0 : L - m)];
-          a.out[0][line * Mx + m] = C2<T>::make(half * (Gp.x + Gc.x) * sc, half * (Gp.y - Gc.y) * sc);
-          a.out[1][line * Mx + m] = C2<T>::make(half * (Gp.y + Gc.y) * sc, -half * (Gp.x - Gc.x) * sc);
+          a.out[0][line * MX + m] = C2<T>::make(half * (Gp.x + Gc.x) * sc, half * (Gp.y - Gc.y) * sc);
+          a.out[1][line * MX + m] = C2<T>::make(half * (Gp.y + Gc.y) * sc, -half * (Gp.x - Gc.x) * sc);
         }
-      } else {
-        for (int m = lane; m < Mx; m += 64) {
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < MP; ++u) {
+        const int m = lane + 64 * u;
+        if (m < MX) {
           const V g = b0[spad(m)];
-          a.out[2][line * Mx + m] = C2<T>::make(g.x * sc, g.y * sc);
+          a.out[2][line * MX + m] = C2<T>::make(g.x * sc, g.y * sc);
         }
       }
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
 template <int L, int RA, int RB, int RC, class V>
 static hipError_t run_xpass_st(Ctx& c, const RegX<V>& a) {
+  if (a.Mx * 3 != L) return hipErrorInvalidValue;  // the kernel's row length is L / 3
   constexpr int NL = sizeof(V) == 16 ? 2 : 4;  // lines per block
   constexpr int PL = L + (L >> 4) + 1;
   const long blocks = (a.nlines + NL - 1) / NL;
