@@ -30,17 +30,19 @@ struct SynthP {
   using TC = TCc;
   static constexpr int NACC = 2, NB = 2;
   static constexpr bool kRecurrence = false;
+  static constexpr bool kCentre = true;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
   const double* rtab;  // FP64 cosines for the recurrence form
   int n4;        // 4N
   int f;         // grid factor
-  MfmaShape sh;  // M = folded points, K = ne
+  MfmaShape sh;  // M = folded points (without the centre row when `centre`), K = ne
   const TI* inf[6];
   TO* outf[6];
   const TI* in;
   TO* out;
   int Ny, P;
+  int centre;    // P odd and the centre point y = 0 summed outside the matrix cores
   long ncold;
   __device__ void select(int z) {
     in = pick6(inf, z);
@@ -60,6 +62,8 @@ struct SynthP {
     const int q = P - 1 - r;
     if (q != r) out[q * ncold + col] = (TO)(acc[0] - acc[1]);
   }
+  // y = 0: T_l(0) = (-1)^(l/2) for even l, 0 for odd l
+  __device__ void store_centre(long col, TC v) const { out[(long)(P / 2) * ncold + col] = (TO)v; }
 };
 
 template <class TI, class TO, class TC>
@@ -70,7 +74,10 @@ static hipError_t synth_t(Ctx& c, const TC* costab, int f, int Ph, int P, const 
   pb.rtab = c.costab;
   pb.n4 = 4 * c.d.N;
   pb.f = f;
-  pb.sh = mfma_shape(Ph, c.d.ne);
+  // the padded grid's centre row would be a lone row of the last 16-row block
+  // (row blocks 13 -> 12 at N = 192, 3 per wave)
+  pb.centre = (P & 1) && P == c.d.Py;
+  pb.sh = mfma_shape(pb.centre ? Ph - 1 : Ph, c.d.ne);
   for (int i = 0; i < nf; ++i) {
     pb.inf[i] = in[i];
     pb.outf[i] = out[i];
@@ -118,6 +125,7 @@ struct AnalP {
   using TC = TCc;
   static constexpr int NACC = 2, NB = 2;
   static constexpr bool kRecurrence = true;
+  static constexpr bool kCentre = false;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
   const double* rtab;  // FP64 cosines for the recurrence form
@@ -282,6 +290,7 @@ struct RhsSynP {
   using TC = double;
   static constexpr int NACC = 2, NB = 2;
   static constexpr bool kRecurrence = false;
+  static constexpr bool kCentre = false;
   static constexpr int bsel(int m) { return m; }
   const double* costab;
   int n4;

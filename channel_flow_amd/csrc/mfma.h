@@ -158,6 +158,7 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
   constexpr int PER = (KC * BN + 255) / 256;  // tile elements per thread
   constexpr bool REC = cos_recurrence<P>();
   __shared__ TC sB[NB][KC][SB];
+  static_assert(NB * KC * SB >= 256 && 256 % BN == 0, "centre-row reduction layout");
   extern __shared__ __align__(16) unsigned char stabraw[];
   TC* stab = reinterpret_cast<TC*>(stabraw);  // 4N cosines (table form)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
       }
     }
   };
+  TC cen = 0;  // this thread's part of the centre-row sum (kCentre problems)
   fetch(0);
   for (int kc = 0; kc < nkc; ++kc) {
 #pragma unroll
@@ -231,6 +233,19 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
     }
     __syncthreads();
     if (kc + 1 < nkc) fetch(kc + 1);
+    if constexpr (P::kCentre) {
+      // centre row: alternating sum of the even-stream rows (KC is even, so the
+      // sign of row kk is (-1)^kk); every thread keeps one column (256 % BN == 0)
+      if (p.centre) {
+#pragma unroll
+        for (int q = 0; q < CB; ++q) {
+          const int i = threadIdx.x + q * 256;
+          const int kk = i / BN, c = i - kk * BN;
+          const TC v = sB[0][kk][c];
+          cen = (kk & 1) ? cen - v : cen + v;
+        }
+      }
+    }
 #pragma unroll
     for (int kb = 0; kb < KC / 4; ++kb) {
       const int kg = kc * (KC / 4) + kb;
@@ -266,6 +281,20 @@ __global__ __launch_bounds__(256) void k_mfma_cos(P pb) {
       }
     }
     __syncthreads();
+  }
+  if constexpr (P::kCentre) {
+    if (p.centre) {
+      // reduce the 256 / BN partial sums of each column through the (now idle) B tile
+      TC* red = &sB[0][0][0];
+      red[threadIdx.x] = cen;
+      __syncthreads();
+      if (threadIdx.x < BN) {
+        TC v = 0;
+        for (int j = 0; j < 256 / BN; ++j) v += red[threadIdx.x + j * BN];
+        const long col = col0 + threadIdx.x;
+        if (col < p.ncold) p.store_centre(col, v);
+      }
+    }
   }
 #pragma unroll
   for (int r = 0; r < RBW; ++r) {
