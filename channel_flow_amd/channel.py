@@ -186,6 +186,13 @@ class Channel:
                 out[self._lib.chf_kernel_name(i).decode()] = dict(choice=cb, ms=[round(x, 4) for x in ms])
         return out
 
+    def set_tile_choice(self, name: str, cb: int):
+        """Overrides the autotuner's choice for one tuned contraction."""
+        names = [self._lib.chf_kernel_name(i).decode() for i in range(self._lib.chf_kernel_count())]
+        if name not in names:
+            raise ValueError(f"unknown kernel class {name!r}")
+        self._check(self._lib.chf_set_tile_choice(self._ctx, names.index(name), int(cb)), "chf_set_tile_choice")
+
     def device_bytes(self) -> int:
         return int(self._lib.chf_device_bytes(self._ctx))
 

@@ -88,29 +88,6 @@ hipError_t dalloc(Ctx* c, T** p, size_t n) {
   return e;
 }
 
-// Packs NACC matrices A_m(r, k) (M x K) into MFMA A-operand fragment order
-// (frag[rb][kb][lane] = A_m[16 rb + lane%16][4 kb + lane/16]) and uploads them.
-hipError_t pack_op(Ctx* c, ChebOp& op, int M, int K, int nacc,
-                   const std::function<double(int, int, int)>& A) {
-  op.sh = mfma_shape(M, K);
-  const size_t n = (size_t)op.sh.MB * op.sh.KB * 64;
-  std::vector<double> f(n);
-  for (int m = 0; m < nacc; ++m) {
-    std::fill(f.begin(), f.end(), 0.0);
-    for (int rb = 0; rb < op.sh.MB; ++rb)
-      for (int kb = 0; kb < op.sh.KB; ++kb)
-        for (int lane = 0; lane < 64; ++lane) {
-          const int r = rb * 16 + (lane & 15), k = kb * 4 + (lane >> 4);
-          if (r < M && k < K) f[((size_t)rb * op.sh.KB + kb) * 64 + lane] = A(m, r, k);
-        }
-    hipError_t e = dalloc(c, &op.frag[m], n);
-    if (e != hipSuccess) return e;
-    e = hipMemcpy(op.frag[m], f.data(), n * sizeof(double), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
-
 int validate(const chf_params* p, std::string& msg) {
   if (!p) { msg = "null parameters"; return -1; }
   if (p->nnx < 4 || p->nnx % 4) { msg = "nnx must be a positive multiple of 4 (PF:721-765 padding)"; return -1; }
@@ -138,7 +115,7 @@ void free_all(Ctx* c) {
     if (p) (void)hipFree(p);
   };
   f(c->alpha); f(c->beta);
-  f(c->v); f(c->oy); f(c->u); f(c->w); f(c->ox); f(c->oz);
+  f(c->v); f(c->vst); f(c->oy); f(c->u); f(c->w); f(c->ox); f(c->oz);
   for (int i = 0; i < 2; ++i) { f(c->hv[i]); f(c->hg[i]); f(c->cl1[i]); f(c->cl3[i]); }
   f(c->au0); f(c->w0);
   for (auto& y : c->Y) f(y);
@@ -368,6 +345,7 @@ int build(Ctx* c) {
   CHF_TRY(dalloc(c, &c->alpha, d.Mx));
   CHF_TRY(dalloc(c, &c->beta, d.Mz));
   CHF_TRY(dalloc(c, &c->v, fld));
+  CHF_TRY(dalloc(c, &c->vst, (size_t)d.ngroups * d.Ny * 2));
   CHF_TRY(dalloc(c, &c->oy, fld));
   CHF_TRY(dalloc(c, &c->u, fld));
   CHF_TRY(dalloc(c, &c->w, fld));
@@ -543,6 +521,7 @@ int build(Ctx* c) {
     CHF_TRY(hipMemsetAsync(q, 0, fld * sizeof(double2), c->stream));
   for (double2* q : {c->cl1[0], c->cl1[1], c->cl3[0], c->cl3[1], c->au0, c->w0})
     CHF_TRY(hipMemsetAsync(q, 0, d.Ny * sizeof(double2), c->stream));
+  CHF_TRY(hipMemsetAsync(c->vst, 0, (size_t)d.ngroups * d.Ny * 2 * sizeof(double2), c->stream));
 
   // kernels that stage more than 64 KiB of LDS
   CHF_TRY(launch_setup_inverses(*c));
@@ -869,6 +848,16 @@ int chf_tile_choice(void* ctx, int id, double* ms_by_choice) {
   if (ms_by_choice)
     for (int i = 0; i < 3; ++i) ms_by_choice[i] = c->tune_ms[id][i];
   return c->cb_choice[id];
+}
+
+int chf_set_tile_choice(void* ctx, int id, int cb) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (id < 0 || id >= K_COUNT || c->cb_choice[id] <= 0)
+    return fail(c, "chf_set_tile_choice: kernel class has no tile choice");
+  if (cb != 1 && cb != 2 && cb != 4) return fail(c, "chf_set_tile_choice: choice must be 1, 2 or 4");
+  c->cb_choice[id] = cb;
+  return 0;
 }
 
 int chf_kernel_count(void) { return K_COUNT; }

@@ -28,7 +28,7 @@ namespace chf {
 template <class TI, class TO, class TCc>
 struct SynthP {
   using TC = TCc;
-  static constexpr int NACC = 2, NB = 2;
+  static constexpr int NACC = 2, NB = 2, KC = 16;  // KC: k rows per B tile
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = true;
   static constexpr int bsel(int m) { return m; }
@@ -62,7 +62,9 @@ struct SynthP {
     const int q = P - 1 - r;
     if (q != r) out[q * ncold + col] = (TO)(acc[0] - acc[1]);
   }
-  // y = 0: T_l(0) = (-1)^(l/2) for even l, 0 for odd l
+  // y = 0: T_l(0) = (-1)^(l/2) for even l, 0 for odd l; even-stream row kk of
+  // tile kc is l = 2 (KC kc + kk), KC even
+  static __device__ TC centre_sign(int kk) { return (kk & 1) ? (TC)-1 : (TC)1; }
   __device__ void store_centre(long col, TC v) const { out[(long)(P / 2) * ncold + col] = (TO)v; }
 };
 
@@ -74,9 +76,9 @@ static hipError_t synth_t(Ctx& c, const TC* costab, int f, int Ph, int P, const 
   pb.rtab = c.costab;
   pb.n4 = 4 * c.d.N;
   pb.f = f;
-  // the padded grid's centre row would be a lone row of the last 16-row block
-  // (row blocks 13 -> 12 at N = 192, 3 per wave)
-  pb.centre = (P & 1) && P == c.d.Py;
+  // with P odd the centre row would be a lone row of the last 16-row block
+  // (N = 192: padded grid 13 -> 12 row blocks, Gauss-Lobatto points 7 -> 6)
+  pb.centre = P & 1;
   pb.sh = mfma_shape(pb.centre ? Ph - 1 : Ph, c.d.ne);
   for (int i = 0; i < nf; ++i) {
     pb.inf[i] = in[i];
@@ -123,7 +125,7 @@ hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const
 template <class TI, class TO, class TCc>
 struct AnalP {
   using TC = TCc;
-  static constexpr int NACC = 2, NB = 2;
+  static constexpr int NACC = 2, NB = 2, KC = 16;  // KC: k rows per B tile
   static constexpr bool kRecurrence = true;
   static constexpr bool kCentre = false;
   static constexpr int bsel(int m) { return m; }
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(256) void k_rhs_prep(PrepArgs a) {
 template <int POFF>
 struct RhsSynP {
   using TC = double;
-  static constexpr int NACC = 2, NB = 2;
+  static constexpr int NACC = 2, NB = 2, KC = 16;  // KC: k rows per B tile
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = false;
   static constexpr int bsel(int m) { return m; }

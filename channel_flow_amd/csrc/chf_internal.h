@@ -10,6 +10,15 @@
 
 namespace chf {
 
+// Workgroups are dealt round-robin to the eight XCDs (block b to XCD b mod 8),
+// each XCD with its own L2.  xcd_order(b, nb) is a permutation of 0..nb-1 that
+// gives every XCD a contiguous run of logical indices, for any block count, so
+// blocks that touch neighbouring data share an L2.
+__device__ __forceinline__ long xcd_order(long b, long nb) {
+  const long q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Sizes derived from chf_params (reference symbols in brackets).
 struct Dims {
   int Mx;     // streamwise modes kept, nnx/2 [nhx+1]
@@ -57,12 +66,6 @@ inline MfmaShape mfma_shape(int M, int K) {
   return s;
 }
 
-// A wall-normal operator packed as MFMA A-operand fragments, one per accumulator.
-struct ChebOp {
-  double* frag[6] = {};
-  MfmaShape sh{};
-};
-
 // Kernel classes for per-launch timing.
 enum KernelId {
   K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
@@ -79,6 +82,7 @@ struct Ctx {
   double* beta = nullptr;   // [Mz]
   // state (complex, [Ny][Mz][Mx] unless noted)
   double2 *v = nullptr, *oy = nullptr;              // coefficients
+  double2* vst = nullptr;  // v as the solve writes it: per (kx, |kz|) group, [Ny][2 sides]
   double2 *u = nullptr, *w = nullptr, *ox = nullptr, *oz = nullptr;  // derived
   double2* hv[2] = {nullptr, nullptr};  // h_v point values; [hcur] = h^n
   double2* hg[2] = {nullptr, nullptr};  // h_g point values

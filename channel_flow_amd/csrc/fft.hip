@@ -705,7 +705,7 @@ __global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(RegZ<V> a) {
   const int Mz = a.Mz, h = Mz / 2;
   // kx groups in XCD-contiguous order: the groups that share 128-byte lines of
   // narrow (FP32) rows go through the same L2
-  const int kg = (gridDim.x & 7) ? (int)blockIdx.x : (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+  const int kg = (int)xcd_order(blockIdx.x, gridDim.x);
   const int kx = kg * TX + tx;
   const int p = blockIdx.y;
   const V* in = a.in[blockIdx.z] + (long)p * a.in_plane;

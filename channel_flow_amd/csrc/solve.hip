@@ -180,7 +180,7 @@ struct SolveArgs {
   const double2 *rvE, *rvO, *roE, *roO;  // parity-folded right-hand sides
   const double* inv;
   const double* inv_w;
-  double2 *v, *oy;
+  double2 *vst, *oy;  // v by group (record [l][side], see KinArgs), omega_y
   double2 *au0, *w0;
   const double2 *c1n, *c1p, *c3n, *c3p;  // mean Lamb point values, n and n-1
   const double *t0, *t2;
@@ -191,13 +191,10 @@ struct SolveArgs {
   double dt, re;
 };
 
-// Block -> (kx, |kz|) group.  Consecutive blocks are dealt round-robin to the
-// eight XCDs, each with its own L2; the remap gives every XCD a contiguous run
-// of groups, so the neighbouring kx columns whose 16-byte values share a cache
-// line are read and written through the same L2.
-__device__ __forceinline__ int xcd_group(int b, int nb) {
-  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
-}
+// Block -> solve block in XCD-contiguous order: every XCD takes a contiguous
+// run of blocks, so the neighbouring kx columns whose 16-byte values share a
+// cache line are read and written through the same L2.
+__device__ __forceinline__ int xcd_group(int b, int nb) { return (int)xcd_order(b, nb); }
 
 // 16-byte load of inverse data that is read once per step: marked non-temporal
 // so the 4.9 GB stream does not evict the right-hand sides and fields from L2.
@@ -214,8 +211,12 @@ __device__ __forceinline__ double2 stream_load(const double* p) {
 // modes) of each of its groups, and writes v^{n+1} (and omega_y^{n+1} when that
 // operator is not diagonalised) (PF:242-394).  The block holding group 0 is
 // the mean mode (PF:401-474): U and W with the omega_y-type operator at k^2 = 0.
-__global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
-  extern __shared__ double rb[];  // [slot][4 vec][Ny], slot = group * neq + eq
+// SU matrix loads in flight per lane in the streaming loop of k_solve.
+constexpr int kSolveSU = 8;
+
+__global__ __launch_bounds__(512) void k_solve(SolveArgs a) {
+  extern __shared__ double rb[];  // [slot][Ny][4 vec], slot = group * neq + eq
+  constexpr int SU = kSolveSU;
   const int Ny = a.Ny, N = a.N, ne = a.ne, no = a.no;
   const int b = xcd_group(blockIdx.x, gridDim.x);
   const int mat = a.bmat[b];
@@ -224,6 +225,26 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
   const bool mean = (gs0 == 0);
   const int neq = a.wdiag ? 1 : 2;  // equations this kernel solves for a non-mean group
   const double c = a.dt * 0.5 / a.re;
+
+  // ---- this thread's task: one row pair (2i, 2i+1) of one half system of one
+  // equation (the launch has at least as many threads as tasks).  Its first SU
+  // matrix columns are requested before the right-hand-side gather, so the
+  // gather's latency overlaps the start of the stream.
+  const int neP = ne + (ne & 1), noP = no + (no & 1);
+  const int pe = neP / 2, po = noP / 2;  // row pairs per half system
+  const int ntask = (mean ? 1 : neq) * (pe + po);
+  const int t = threadIdx.x;
+  const bool active = t < ntask;
+  const int eq = mean ? 1 : t / (pe + po);
+  const int rr = mean ? t : t - eq * (pe + po);
+  const bool odd = rr >= pe;
+  const int i2 = 2 * (odd ? rr - pe : rr);
+  const int n = odd ? no : ne;
+  const long nP = eq ? a.row_w : a.row;
+  const double* M = (eq ? a.inv_w + (long)mat * a.stride_w : a.inv + (long)mat * a.stride) + (odd ? neP : 0) + i2;
+  double2 mb[SU];
+#pragma unroll
+  for (int u = 0; u < SU; ++u) mb[u] = (active && u < n) ? stream_load(M + (long)u * nP) : make_double2(0, 0);
 
   // ---- right-hand sides, parity-folded (rows of the half systems)
   if (!mean) {
@@ -244,11 +265,11 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
           x0 = src[(long)r * a.ncol + col0];
           if (kzab > 0) x1 = src[(long)r * a.ncol + col1];
         }
-        double* dst = rb + (gi * neq + eq) * 4 * Ny;
-        dst[0 * Ny + row] = x0.x;
-        dst[1 * Ny + row] = x0.y;
-        dst[2 * Ny + row] = x1.x;
-        dst[3 * Ny + row] = x1.y;
+        double* dst = rb + ((gi * neq + eq) * Ny + row) * 4;
+        dst[0] = x0.x;
+        dst[1] = x0.y;
+        dst[2] = x1.x;
+        dst[3] = x1.y;
       }
     }
   } else {
@@ -280,60 +301,62 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
           acc[2] += 0.5 * (Gp.x + s * Gq.x); acc[3] += 0.5 * (Gp.y + s * Gq.y);
         }
       }
-      for (int q4 = 0; q4 < 4; ++q4) rb[q4 * Ny + row] = acc[q4];  // slot 0
+      for (int q4 = 0; q4 < 4; ++q4) rb[row * 4 + q4] = acc[q4];  // slot 0
     }
   }
   __syncthreads();
-  // ---- dense half-system inverses x = Einv be, Oinv bo.  Thread = one row pair
-  // (2i, 2i+1) of one half system (of one equation), 16-byte matrix loads,
+  // ---- dense half-system inverses x = Einv be, Oinv bo: 16-byte matrix loads,
   // accumulating the vectors of every group of the block.
-  const int neP = ne + (ne & 1), noP = no + (no & 1);
-  const int pe = neP / 2, po = noP / 2;  // row pairs per half system
-  const int ntask = (mean ? 1 : neq) * (pe + po);
-  for (int t = threadIdx.x; t < ntask; t += blockDim.x) {
-    const int eq = mean ? 1 : t / (pe + po);
-    const int rr = mean ? t : t - eq * (pe + po);
-    const bool odd = rr >= pe;
-    const int i2 = 2 * (odd ? rr - pe : rr);
-    const int n = odd ? no : ne;
-    const long nP = eq ? a.row_w : a.row;
-    const double* M = (eq ? a.inv_w + (long)mat * a.stride_w : a.inv + (long)mat * a.stride) + (odd ? neP : 0) + i2;
+  if (active) {
     const int ns = mean ? 1 : ngr;  // vector sets (slots) for this equation
-    const double* rhs0 = rb + (mean ? 0 : eq) * 4 * Ny + (odd ? ne : 0);
-    const double* rhs1 = rb + (neq + eq) * 4 * Ny + (odd ? ne : 0);
+    const double* rhs0 = rb + ((mean ? 0 : eq) * Ny + (odd ? ne : 0)) * 4;
+    const double* rhs1 = rb + ((neq + eq) * Ny + (odd ? ne : 0)) * 4;
     double x[2][8] = {};
-    int j = 0;
-    for (; j + 4 <= n; j += 4) {
-      double2 m4[4];
+    // The right-hand sides are the same for every lane (LDS broadcasts); a
+    // scheduling barrier per column keeps their reads next to their use, so
+    // they hold registers for one column only and the load depth, not the
+    // right-hand sides, fills the registers.
+    auto column = [&](const double2 m, int jj) {
+      __builtin_amdgcn_sched_barrier(0);
+      const double* r0 = rhs0 + jj * 4;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) m4[u] = stream_load(M + (long)(j + u) * nP);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int q = 0; q < 4; ++q) {
+        x[0][q] = fma(m.x, r0[q], x[0][q]);
+        x[0][4 + q] = fma(m.y, r0[q], x[0][4 + q]);
+      }
+      if (ns > 1) {
+        const double* r1 = rhs1 + jj * 4;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const double r = rhs0[q * Ny + j + u];
-          x[0][q] = fma(m4[u].x, r, x[0][q]);
-          x[0][4 + q] = fma(m4[u].y, r, x[0][4 + q]);
-        }
-        if (ns > 1) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const double r = rhs1[q * Ny + j + u];
-            x[1][q] = fma(m4[u].x, r, x[1][q]);
-            x[1][4 + q] = fma(m4[u].y, r, x[1][4 + q]);
-          }
+          x[1][q] = fma(m.x, r1[q], x[1][q]);
+          x[1][4 + q] = fma(m.y, r1[q], x[1][4 + q]);
         }
       }
+    };
+    int j = 0;
+    if (n < SU) {
+#pragma unroll
+      for (int u = 0; u < SU; ++u)
+        if (u < n) column(mb[u], u);
+      j = n;
+    }
+    for (; j + SU <= n; j += SU) {
+      if (j > 0) {
+#pragma unroll
+        for (int u = 0; u < SU; ++u) mb[u] = stream_load(M + (long)(j + u) * nP);
+      }
+#pragma unroll
+      for (int u = 0; u < SU; ++u) column(mb[u], j + u);
     }
     for (; j < n; ++j) {
       const double2 m = stream_load(M + (long)j * nP);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const double r = rhs0[q * Ny + j];
+        const double r = rhs0[j * 4 + q];
         x[0][q] = fma(m.x, r, x[0][q]);
         x[0][4 + q] = fma(m.y, r, x[0][4 + q]);
         if (ns > 1) {
-          const double r1 = rhs1[q * Ny + j];
+          const double r1 = rhs1[j * 4 + q];
           x[1][q] = fma(m.x, r1, x[1][q]);
           x[1][4 + q] = fma(m.y, r1, x[1][4 + q]);
         }
@@ -352,10 +375,15 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs a) {
         if (mean) {
           a.au0[l] = make_double2(xx[4 * h + 0], xx[4 * h + 1]);
           a.w0[l] = make_double2(xx[4 * h + 2], xx[4 * h + 3]);
+        } else if (eq == 0) {
+          // v: the group's record is written by this block alone, in full
+          // lines (scattered 16-byte column stores cost a third of the solve)
+          double2* rec = a.vst + (long)g * Ny * 2;
+          rec[2 * l] = make_double2(xx[4 * h + 0], xx[4 * h + 1]);
+          if (kzab > 0) rec[2 * l + 1] = make_double2(xx[4 * h + 2], xx[4 * h + 3]);
         } else {
-          double2* dst = eq ? a.oy : a.v;
-          dst[(long)l * a.ncol + col0] = make_double2(xx[4 * h + 0], xx[4 * h + 1]);
-          if (kzab > 0) dst[(long)l * a.ncol + col1] = make_double2(xx[4 * h + 2], xx[4 * h + 3]);
+          a.oy[(long)l * a.ncol + col0] = make_double2(xx[4 * h + 0], xx[4 * h + 1]);
+          if (kzab > 0) a.oy[(long)l * a.ncol + col1] = make_double2(xx[4 * h + 2], xx[4 * h + 3]);
         }
       }
     }
@@ -507,6 +535,35 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
   }
 }
 
+// v from the solve's per-group records (record g = |kz| Mx + kx holds
+// [l][side], side 0 for kz = |kz|, 1 for kz = Mz - |kz|) into the field layout
+// [l][kz][kx].  Block = one |kz| and VL degrees: it reads the 256-byte record
+// segments of its Mx groups whole and writes Mx-long rows of both kz.  The
+// mean column and the span Nyquist row are not written.
+constexpr int kVL = 8;
+
+__global__ __launch_bounds__(256) void k_vtranspose(const double2* __restrict__ vst, double2* __restrict__ v,
+                                                    int Mx, int Mz, int Ny, long ncol) {
+  extern __shared__ double2 tile[];  // [VL][2][Mx]
+  const int kzab = blockIdx.x, l0 = blockIdx.y * kVL;
+  const int nl = min(kVL, Ny - l0);
+  const int per = 2 * kVL;  // double2 per record segment
+  for (int i = threadIdx.x; i < Mx * per; i += blockDim.x) {
+    const int kx = i / per, e = i - kx * per;
+    const int l = e >> 1, side = e & 1;
+    if (l < nl) tile[(l * 2 + side) * Mx + kx] = vst[((long)(kzab * Mx + kx) * Ny + l0 + l) * 2 + side];
+  }
+  __syncthreads();
+  const int nside = kzab ? 2 : 1;
+  for (int i = threadIdx.x; i < nl * nside * Mx; i += blockDim.x) {
+    const int kx = i % Mx, ls = i / Mx;
+    const int l = ls / nside, side = ls - l * nside;
+    const int kz = side ? Mz - kzab : kzab;
+    if (kz == 0 && kx == 0) continue;  // the mean column: U, W are solved instead
+    v[(long)(l0 + l) * ncol + (long)kz * Mx + kx] = tile[(l * 2 + side) * Mx + kx];
+  }
+}
+
 hipError_t launch_solve(Ctx& c) {
   const Dims& d = c.d;
   SolveArgs a{};
@@ -516,7 +573,7 @@ hipError_t launch_solve(Ctx& c) {
   a.roO = c.roO;
   a.inv = c.inv;
   a.inv_w = c.inv_w;
-  a.v = c.v;
+  a.vst = c.vst;
   a.oy = c.oy;
   a.au0 = c.au0;
   a.w0 = c.w0;
@@ -549,9 +606,19 @@ hipError_t launch_solve(Ctx& c) {
     hipError_t e = hipFuncSetAttribute((const void*)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
-  // with the omega_y operator diagonalised a block streams the two v half
-  // systems only: ne/2 + no/2 row pairs, so 128 threads
-  hipLaunchKernelGGL(k_solve, dim3(d.nblocks), dim3(c.wdiag ? 128 : 256), shm, c.stream, a);
+  // one thread per task: with the omega_y operator diagonalised a block streams
+  // the two v half systems only (ne/2 + no/2 row pairs, 128 threads at N = 192),
+  // otherwise the omega_y half systems as well
+  const int pairs = (d.ne + (d.ne & 1)) / 2 + (d.no + (d.no & 1)) / 2;
+  const int threads = ((c.wdiag ? 1 : 2) * pairs + 63) / 64 * 64;
+  if (threads > 512) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_solve, dim3(d.nblocks), dim3(threads), shm, c.stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t tshm = sizeof(double2) * 2ull * kVL * d.Mx;
+  if (tshm > 65536) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vtranspose, dim3(d.Mz / 2, (d.Ny + kVL - 1) / kVL), dim3(256), tshm, c.stream,
+                     (const double2*)c.vst, c.v, d.Mx, d.Mz, d.Ny, d.ncol);
   return hipGetLastError();
 }
 

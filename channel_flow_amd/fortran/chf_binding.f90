@@ -141,6 +141,13 @@ module chf_binding
       integer(c_int), value :: id
       real(c_double), intent(out) :: ms_by_choice(3)
     end function chf_tile_choice
+
+    ! override of the tuned choice (1, 2 or 4); results do not change
+    integer(c_int) function chf_set_tile_choice(ctx, id, cb) bind(C, name="chf_set_tile_choice")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: id, cb
+    end function chf_set_tile_choice
   end interface
 
 contains

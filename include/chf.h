@@ -119,6 +119,9 @@ int chf_timing(void* ctx, int id, double* total_ms, int* launches);
  * without a tile choice); ms_by_choice (may be NULL) receives the three
  * measured times per launch.  The choice never changes results. */
 int chf_tile_choice(void* ctx, int id, double* ms_by_choice);
+/* Overrides the tuned choice of kernel class `id` (a class for which
+ * chf_tile_choice returns > 0) with cb in {1, 2, 4}; results do not change. */
+int chf_set_tile_choice(void* ctx, int id, int cb);
 
 /* Ensemble: n independent channels (any mix of grid sizes, Reynolds numbers
  * and precisions) advanced together, one step of every member being one

@@ -270,3 +270,25 @@ def test_autotuner_reports_choices(require_gpu):
     for name, t in tiles.items():
         assert t["choice"] in (1, 2, 4), (name, t)
         assert all(ms > 0 for ms in t["ms"]), (name, t)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tile_choices_bit_identical(require_gpu, precision):
+    """Every column-block choice of the tuned contractions (forced through
+    chf_set_tile_choice) gives bit-identical states after initial() and two
+    steps, at N = 192 (row blocks that admit 1, 2 and 4 column blocks, the
+    centre rows summed outside the matrix cores)."""
+    cfg = dict(nnx=32, nny=32, nnz=192, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
+               dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=24)
+    v, oy, au0, w0 = ic_of(cfg)
+    states = {}
+    for cb in (1, 2, 4):
+        with make_channel(cfg, precision) as ch:
+            for name in ch.tile_choices():
+                ch.set_tile_choice(name, cb)
+            ch.initial(v, oy, au0, w0)
+            ch.step(2)
+            states[cb] = ch.get_state()
+    for cb in (2, 4):
+        for k in STATE_KEYS:
+            np.testing.assert_array_equal(states[cb][k], states[1][k], err_msg=f"{precision} cb={cb} {k}")
