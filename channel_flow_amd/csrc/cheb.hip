@@ -52,10 +52,17 @@ struct SynthP {
     return (int)(((long)f * row * (2 * k0 + m)) % n4_);
   }
   __device__ int astep(int, int row, int n4_) const { return (int)((8L * f * row) % n4_); }
-  __device__ void loadB(int j, long col, TC* v) const {
+  // B streams: even and odd coefficients (raw loads; converted when staged)
+  using TL = TI;
+  static constexpr int NR = 2;
+  __device__ void fetchB(int j, long col, TL* r) const {
     const int l = 2 * j;
-    v[0] = (TC)in[l * ncold + col];
-    v[1] = (l + 1 < Ny) ? (TC)in[(l + 1) * ncold + col] : (TC)0;
+    r[0] = in[l * ncold + col];
+    r[1] = (l + 1 < Ny) ? in[(l + 1) * ncold + col] : (TL)0;
+  }
+  __device__ void makeB(int, const TL* r, TC* v) const {
+    v[0] = (TC)r[0];
+    v[1] = (TC)r[1];
   }
   __device__ void store(int r, long col, const TC* acc) const {
     out[r * ncold + col] = (TO)(acc[0] + acc[1]);
@@ -148,15 +155,24 @@ struct AnalP {
     return (int)(((long)(2 * row + m) * k0) % n4_);
   }
   __device__ int astep(int m, int row, int n4_) const { return (int)((4L * (2 * row + m)) % n4_); }
-  __device__ void loadB(int p, long col, TC* v) const {
-    const TC x = (TC)in[p * ncold + col];
+  // B streams: weighted folded sums w_p (x_p +- x_{P-1-p}); the loads are
+  // issued ahead, the folding is done when the tile is staged
+  using TL = TI;
+  static constexpr int NR = 2;
+  __device__ void fetchB(int p, long col, TL* r) const {
+    const int q = P - 1 - p;
+    r[0] = in[p * ncold + col];
+    r[1] = (q != p) ? in[q * ncold + col] : (TL)0;
+  }
+  __device__ void makeB(int p, const TL* r, TC* v) const {
+    const TC x = (TC)r[0];
     const int q = P - 1 - p;
     const TC w = (p == 0) ? (TC)1 : (TC)2;
     if (q == p) {
       v[0] = w * x;
       v[1] = 0;
     } else {
-      const TC y = (TC)in[q * ncold + col];
+      const TC y = (TC)r[1];
       v[0] = w * (x + y);
       v[1] = w * (x - y);
     }
@@ -308,9 +324,15 @@ struct RhsSynP {
     return (int)((2L * (row + POFF) * (2 * k0 + m)) % n4_);   // T_l(y_p) = cos(pi p l / N)
   }
   __device__ int astep(int, int row, int n4_) const { return (int)((16L * (row + POFF)) % n4_); }
-  __device__ void loadB(int j, long col, double* v) const {
-    v[0] = g[(2 * j) * ncold + col];
-    v[1] = (2 * j + 1 <= N) ? g[(2 * j + 1) * ncold + col] : 0.0;
+  using TL = double;
+  static constexpr int NR = 2;
+  __device__ void fetchB(int j, long col, TL* r) const {
+    r[0] = g[(2 * j) * ncold + col];
+    r[1] = (2 * j + 1 <= N) ? g[(2 * j + 1) * ncold + col] : 0.0;
+  }
+  __device__ void makeB(int, const TL* r, double* v) const {
+    v[0] = r[0];
+    v[1] = r[1];
   }
   __device__ void store(int r, long col, const double* acc) const {
     const int p = r + POFF, q = N - p;

@@ -4,8 +4,10 @@
 //
 // A_m: [M x K] matrices whose entries are cosines cos(pi n / 2N) (Chebyshev
 //      polynomials at Gauss-Lobatto points), generated per lane;
-// B_s: NB real column streams [K x ncold] produced by Prob::loadB from the
-//      field arrays (parity gathers, mirrored-point folds);
+// B_s: NB real column streams [K x ncold]: Prob::fetchB loads NR raw values
+//      per (k, column) from the field arrays (issued one tile ahead) and
+//      Prob::makeB forms the stream values from them when the tile is staged
+//      (parity gathers, mirrored-point folds, conversion to the compute type);
 // one workgroup owns all M rows of a BN-column tile, so every B element is
 // read from HBM once; wave w of NWV owns row blocks w, w + NWV, w + 2 NWV, ...
 #pragma once
@@ -127,7 +129,11 @@ __global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
 #pragma unroll
       for (int c = 0; c < CB; ++c) acc[m][r][c] = typename MF::acc{0, 0, 0, 0};
   const int nkc = (KB * 4 + KC - 1) / KC;
-  TC nxt[PER][NB];
+  // raw loads of the next tile: nothing consumes them until it is staged, so
+  // they stay in flight across the current tile's matrix-core work
+  using TL = typename P::TL;
+  constexpr int NR = P::NR;
+  TL nxt[PER][NR];
   auto fetch = [&](int kc) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -135,12 +141,9 @@ __global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
       const int kk = i / BN, c = i - kk * BN;
       const int k = kc * KC + kk;
       const long col = col0 + c;
-      if (i < KC * BN && k < p.sh.K && col < p.ncold) {
-        p.loadB(k, col, nxt[q]);
-      } else {
 #pragma unroll
-        for (int s = 0; s < NB; ++s) nxt[q][s] = 0;
-      }
+      for (int s = 0; s < NR; ++s) nxt[q][s] = 0;
+      if (i < KC * BN && k < p.sh.K && col < p.ncold) p.fetchB(k, col, nxt[q]);
     }
   };
   TC cen = 0;  // centre-row sum of this lane's column (kCentre problems)
@@ -151,8 +154,10 @@ __global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
       const int i = threadIdx.x + q * NT;
       if (i < KC * BN) {
         const int kk = i / BN, c = i - kk * BN;
+        TC v[NB];
+        p.makeB(kc * KC + kk, nxt[q], v);
 #pragma unroll
-        for (int s = 0; s < NB; ++s) sB[s][kk][c] = nxt[q][s];
+        for (int s = 0; s < NB; ++s) sB[s][kk][c] = v[s];
       }
     }
     __syncthreads();
