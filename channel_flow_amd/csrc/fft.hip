@@ -393,6 +393,9 @@ struct Geo {
   static constexpr int L = N1 * N2;
   static constexpr int G = N1 > N2 ? N1 : N2;
   static constexpr int BUF_A = N1 * (N2 + 1);  // inverse/first exchange tile
+  // LDS stride between the exchange tiles of the lines of one span-pass block
+  // (paddings of 1, 2 and 8 elements measured: no faster)
+  static constexpr int LINE_Z = BUF_A;
   static constexpr int BUF_B = N2 * (N1 + 1);  // transposed-order tile
   static constexpr int BUF = (BUF_A > BUF_B ? BUF_A : BUF_B) > L ? (BUF_A > BUF_B ? BUF_A : BUF_B) : L;
 };
@@ -701,7 +704,7 @@ __global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(RegZ<V> a) {
   V* sm = reinterpret_cast<V*>(smraw);
   V* tw = sm;
   const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
-  V* buf = sm + L + tx * Gm::BUF_A;
+  V* buf = sm + L + tx * Gm::LINE_Z;
   const int Mz = a.Mz, h = Mz / 2;
   // kx groups in XCD-contiguous order: the groups that share 128-byte lines of
   // narrow (FP32) rows go through the same L2
@@ -788,7 +791,7 @@ static hipError_t run_zfft_reg(Ctx& c, const RegZ<V>& a, bool inv, int nf) {
   using Gm = Geo<N1, N2>;
   constexpr int TX = (256 / Gm::G) >= 16 ? 16 : ((256 / Gm::G) >= 8 ? 8 : 4);
   if (c.d.Mx % TX) return hipErrorInvalidValue;
-  const size_t shm = sizeof(V) * (Gm::L + (size_t)TX * Gm::BUF_A);
+  const size_t shm = sizeof(V) * (Gm::L + (size_t)TX * Gm::LINE_Z);
   dim3 grid(c.d.Mx / TX, c.d.Py, nf);
   const void* fn = inv ? (const void*)k_zfft_reg<N1, N2, TX, true, V> : (const void*)k_zfft_reg<N1, N2, TX, false, V>;
   if (shm > 65536) {

@@ -211,8 +211,9 @@ __device__ __forceinline__ double2 stream_load(const double* p) {
 // modes) of each of its groups, and writes v^{n+1} (and omega_y^{n+1} when that
 // operator is not diagonalised) (PF:242-394).  The block holding group 0 is
 // the mean mode (PF:401-474): U and W with the omega_y-type operator at k^2 = 0.
-// SU matrix loads in flight per lane in the streaming loop of k_solve.
-constexpr int kSolveSU = 8;
+// Matrix columns per load batch in the streaming loop of k_solve; two batches
+// alternate, so the next batch is in flight while the current one is used.
+constexpr int kSolveSU = 4;
 
 __global__ __launch_bounds__(512) void k_solve(SolveArgs a) {
   extern __shared__ double rb[];  // [slot][Ny][4 vec], slot = group * neq + eq
@@ -242,9 +243,9 @@ __global__ __launch_bounds__(512) void k_solve(SolveArgs a) {
   const int n = odd ? no : ne;
   const long nP = eq ? a.row_w : a.row;
   const double* M = (eq ? a.inv_w + (long)mat * a.stride_w : a.inv + (long)mat * a.stride) + (odd ? neP : 0) + i2;
-  double2 mb[SU];
+  double2 ma[SU], mb[SU];
 #pragma unroll
-  for (int u = 0; u < SU; ++u) mb[u] = (active && u < n) ? stream_load(M + (long)u * nP) : make_double2(0, 0);
+  for (int u = 0; u < SU; ++u) ma[u] = (active && u < n) ? stream_load(M + (long)u * nP) : make_double2(0, 0);
 
   // ---- right-hand sides, parity-folded (rows of the half systems)
   if (!mean) {
@@ -333,20 +334,31 @@ __global__ __launch_bounds__(512) void k_solve(SolveArgs a) {
         }
       }
     };
+    // batches [j, j + SU) alternate between ma and mb; the first was requested
+    // before the gather.  The next batch is always requested (its columns
+    // clamped to n - 1), so the number of loads in flight is fixed and the
+    // current batch waits for its own loads only.
     int j = 0;
     if (n < SU) {
 #pragma unroll
       for (int u = 0; u < SU; ++u)
-        if (u < n) column(mb[u], u);
+        if (u < n) column(ma[u], u);
       j = n;
-    }
-    for (; j + SU <= n; j += SU) {
-      if (j > 0) {
+    } else {
+      while (true) {
 #pragma unroll
-        for (int u = 0; u < SU; ++u) mb[u] = stream_load(M + (long)(j + u) * nP);
+        for (int u = 0; u < SU; ++u) mb[u] = stream_load(M + (long)min(j + SU + u, n - 1) * nP);
+#pragma unroll
+        for (int u = 0; u < SU; ++u) column(ma[u], j + u);
+        j += SU;
+        if (j + SU > n) break;
+#pragma unroll
+        for (int u = 0; u < SU; ++u) ma[u] = stream_load(M + (long)min(j + SU + u, n - 1) * nP);
+#pragma unroll
+        for (int u = 0; u < SU; ++u) column(mb[u], j + u);
+        j += SU;
+        if (j + SU > n) break;
       }
-#pragma unroll
-      for (int u = 0; u < SU; ++u) column(mb[u], j + u);
     }
     for (; j < n; ++j) {
       const double2 m = stream_load(M + (long)j * nP);
