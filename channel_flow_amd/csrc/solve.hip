@@ -677,24 +677,52 @@ __global__ __launch_bounds__(512) void k_wsolve(WArgs a) {
   }
   __syncthreads();
   MF::acc acc[TPW][4];
+  // acc = A B over k blocks 0..KB-1 (A: packed fragments of L or R, MB row
+  // blocks; B: the LDS tile).  The A fragments come from L2 in batches of four
+  // k blocks, the next batch requested before the current one is used; the
+  // batch indices are clamped so the loads are unconditional and the number in
+  // flight is fixed.
+  auto product = [&](const double* A, int MB) {
 #pragma unroll
-  for (int t = 0; t < TPW; ++t)
+    for (int t = 0; t < TPW; ++t)
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::acc{0, 0, 0, 0};
-  for (int kb = 0; kb < KB; ++kb) {
-    double bf[4];
+      for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::acc{0, 0, 0, 0};
+    double fa[TPW][4], fb[TPW][4];
+    auto fetch = [&](double (&f)[TPW][4], int kb0) {
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) bf[cb] = sB[(4 * kb + (lane >> 4)) * CT + 16 * cb + (lane & 15)];
+      for (int t = 0; t < TPW; ++t) {
+        const int rb = min(wave + NW * t, MB - 1);
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      const int rb = wave + NW * t;
-      if (rb < MB1) {
-        const double af = L[((long)rb * KB + kb) * 64 + lane];
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::mma(af, bf[cb], acc[t][cb]);
+        for (int u = 0; u < 4; ++u) f[t][u] = A[((long)rb * KB + min(kb0 + u, KB - 1)) * 64 + lane];
       }
+    };
+    auto use = [&](const double (&f)[TPW][4], int kb0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kb = kb0 + u;
+        if (kb >= KB) break;
+        double bf[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) bf[cb] = sB[(4 * kb + (lane >> 4)) * CT + 16 * cb + (lane & 15)];
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+          if (wave + NW * t < MB) {
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::mma(f[t][u], bf[cb], acc[t][cb]);
+          }
+        }
+      }
+    };
+    fetch(fa, 0);
+    for (int kb0 = 0; kb0 < KB; kb0 += 8) {
+      fetch(fb, kb0 + 4);
+      use(fa, kb0);
+      if (kb0 + 4 >= KB) break;
+      fetch(fa, kb0 + 8);
+      use(fb, kb0 + 4);
     }
-  }
+  };
+  product(L, MB1);
   __syncthreads();
   // scale rows by 1 / (1 + c (k^2 - lam)) and write Y back as the next B operand
   double s[4];
@@ -722,24 +750,7 @@ __global__ __launch_bounds__(512) void k_wsolve(WArgs a) {
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int t = 0; t < TPW; ++t)
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::acc{0, 0, 0, 0};
-  for (int kb = 0; kb < KB; ++kb) {
-    double bf[4];
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) bf[cb] = sB[(4 * kb + (lane >> 4)) * CT + 16 * cb + (lane & 15)];
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      const int rb = wave + NW * t;
-      if (rb < MB2) {
-        const double af = R[((long)rb * KB + kb) * 64 + lane];
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb) acc[t][cb] = MF::mma(af, bf[cb], acc[t][cb]);
-      }
-    }
-  }
+  product(R, MB2);
   // store x_i -> omega_y at degree 2i + p; the mean mode and the span Nyquist
   // column are not solved here (PF:247)
 #pragma unroll
