@@ -48,10 +48,11 @@ struct SynthP {
     in = pick6(inf, z);
     out = pick6(outf, z);
   }
+  // table indices in 32-bit unsigned arithmetic (products below 2^14 for N <= 256)
   __device__ int aindex(int m, int row, int k0, int n4_) const {
-    return (int)(((long)f * row * (2 * k0 + m)) % n4_);
+    return (int)((unsigned)(f * row * (2 * k0 + m)) % (unsigned)n4_);
   }
-  __device__ int astep(int, int row, int n4_) const { return (int)((8L * f * row) % n4_); }
+  __device__ int astep(int, int row, int n4_) const { return (int)((unsigned)(8 * f * row) % (unsigned)n4_); }
   // B streams: even and odd coefficients (raw loads; converted when staged)
   using TL = TI;
   static constexpr int NR = 2;
@@ -152,9 +153,9 @@ struct AnalP {
     out = pick6(outf, z);
   }
   __device__ int aindex(int m, int row, int k0, int n4_) const {
-    return (int)(((long)(2 * row + m) * k0) % n4_);
+    return (int)((unsigned)((2 * row + m) * k0) % (unsigned)n4_);
   }
-  __device__ int astep(int m, int row, int n4_) const { return (int)((4L * (2 * row + m)) % n4_); }
+  __device__ int astep(int m, int row, int n4_) const { return (int)((unsigned)(4 * (2 * row + m)) % (unsigned)n4_); }
   // B streams: weighted folded sums w_p (x_p +- x_{P-1-p}); the loads are
   // issued ahead, the folding is done when the tile is staged
   using TL = TI;
@@ -321,9 +322,9 @@ struct RhsSynP {
   double dt;
   __device__ void select(int) {}
   __device__ int aindex(int m, int row, int k0, int n4_) const {
-    return (int)((2L * (row + POFF) * (2 * k0 + m)) % n4_);   // T_l(y_p) = cos(pi p l / N)
+    return (int)((unsigned)(2 * (row + POFF) * (2 * k0 + m)) % (unsigned)n4_);  // T_l(y_p) = cos(pi p l / N)
   }
-  __device__ int astep(int, int row, int n4_) const { return (int)((16L * (row + POFF)) % n4_); }
+  __device__ int astep(int, int row, int n4_) const { return (int)((unsigned)(16 * (row + POFF)) % (unsigned)n4_); }
   using TL = double;
   static constexpr int NR = 2;
   __device__ void fetchB(int j, long col, TL* r) const {
