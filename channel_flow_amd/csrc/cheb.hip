@@ -221,7 +221,7 @@ hipError_t launch_analysis32(Ctx& c, const float2* const* in, double2* const* ou
 
 __device__ __forceinline__ double col_k2(long col, int Mx, const double* al, const double* be) {
   const long cm = col >> 1;
-  const int kz = (int)(cm / Mx);
+  const int kz = (int)((unsigned)cm / (unsigned)Mx);
   const int kx = (int)(cm - (long)kz * Mx);
   return al[kx] * al[kx] + be[kz] * be[kz];
 }
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_rhs_prep(PrepArgs a) {
   const int p = blockIdx.y;
   const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cm >= a.ncol) return;
-  const int kz = (int)(cm / a.Mx);
+  const int kz = (int)((unsigned)cm / (unsigned)a.Mx);
   const int kx = (int)(cm - (long)kz * a.Mx);
   const double s = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
   const double c = a.c, c2 = 1.0 - 2.0 * c * s, c0 = s * (1.0 - c * s), cw = 1.0 - c * s;
@@ -357,7 +357,8 @@ hipError_t launch_rhs_prep(Ctx& c) {
   a.N = c.d.N;
   a.ncol = c.d.ncol;
   a.c = c.prm.dt * 0.5 / c.prm.re;
-  hipLaunchKernelGGL(k_rhs_prep, dim3((unsigned)((a.ncol + 255) / 256), 2), dim3(256), 0, c.stream, a);
+  hipLaunchKernelGGL(k_rhs_prep, dim3((unsigned)((a.ncol + kColumnThreads - 1) / kColumnThreads), 2),
+                     dim3(kColumnThreads), 0, c.stream, a);
   return hipGetLastError();
 }
 
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(256) void k_hassemble(HArgs a, MArgs m) {
   }
   const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cm >= a.ncol) return;
-  const int kz = (int)(cm / a.Mx);
+  const int kz = (int)((unsigned)cm / (unsigned)a.Mx);
   const int kx = (int)(cm - (long)kz * a.Mx);
   const double al = a.al[kx], be = a.be[kz];
   const double k2 = al * al + be * be;
@@ -533,7 +534,8 @@ hipError_t launch_hassemble(Ctx& c) {
   m.Ny = c.d.Ny;
   m.Pn = c.d.Pn;
   m.ncol = c.d.ncol;
-  hipLaunchKernelGGL(k_hassemble, dim3((unsigned)((a.ncol + 255) / 256) + 1), dim3(256), 0, c.stream, a, m);
+  hipLaunchKernelGGL(k_hassemble, dim3((unsigned)((a.ncol + kColumnThreads - 1) / kColumnThreads) + 1),
+                     dim3(kColumnThreads), 0, c.stream, a, m);
   return hipGetLastError();
 }
 

@@ -19,6 +19,11 @@ __device__ __forceinline__ long xcd_order(long b, long nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Threads per block of the one-thread-per-column kernels (derivative sweeps, h
+// assembly, kinematics): one wavefront, so that their 32768 columns at
+// 256x193x256 spread over all 256 CUs (256-thread blocks left half of them idle).
+constexpr int kColumnThreads = 64;
+
 // Sizes derived from chf_params (reference symbols in brackets).
 struct Dims {
   int Mx;     // streamwise modes kept, nnx/2 [nhx+1]

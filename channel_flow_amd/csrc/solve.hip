@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
   }
   const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cm >= a.ncol || cm == 0) return;
-  const int kz = (int)(cm / a.Mx);
+  const int kz = (int)((unsigned)cm / (unsigned)a.Mx);
   const int kx = (int)(cm - (long)kz * a.Mx);
   if (kz == a.Mz / 2) return;  // span Nyquist: never solved, stays zero (PF:247)
   const double al = a.al[kx], be = a.be[kz];
@@ -729,7 +729,7 @@ __global__ __launch_bounds__(512) void k_wsolve(WArgs a) {
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
     const long cm = (col0 + 16 * cb + (lane & 15)) >> 1;
-    const int kz = (int)(cm / a.Mx), kx = (int)(cm - (long)kz * a.Mx);
+    const int kz = (int)((unsigned)cm / (unsigned)a.Mx), kx = (int)(cm - (long)kz * a.Mx);
     const bool in = cm < (a.ncold >> 1);
     const double alx = in ? a.al[kx] : 0.0, bez = in ? a.be[kz] : 0.0;
     s[cb] = alx * alx + bez * bez;
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(512) void k_wsolve(WArgs a) {
     const long col = col0 + 16 * cb + (lane & 15);
     const long cm = col >> 1;
     if (col >= a.ncold || cm == 0) continue;
-    const int kz = (int)(cm / a.Mx);
+    const int kz = (int)((unsigned)cm / (unsigned)a.Mx);
     if (kz == a.Mz / 2) continue;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
@@ -823,14 +823,14 @@ hipError_t launch_kinematics(Ctx& c) {
   a.Ny = d.Ny;
   a.ncol = d.ncol;
   a.c = c.prm.dt * 0.5 / c.prm.re;
-  const unsigned nb = (unsigned)((d.ncol + 255) / 256) + 1;  // + the mean-mode block
+  const unsigned nb = (unsigned)((d.ncol + kColumnThreads - 1) / kColumnThreads) + 1;  // + the mean-mode block
   const size_t shm = sizeof(double) * 8ull * d.Ny;
   if (shm > 65536) {
     hipError_t e =
         hipFuncSetAttribute((const void*)k_kinematics, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_kinematics, dim3(nb, 2), dim3(256), shm, c.stream, a);
+  hipLaunchKernelGGL(k_kinematics, dim3(nb, 2), dim3(kColumnThreads), shm, c.stream, a);
   return hipGetLastError();
 }
 
