@@ -235,8 +235,10 @@ def cpu_baseline(cfg, seconds_budget=30.0):
 
 
 def run_form(cfg, ic_arrays, precision, steps, warmup, total, log):
-    """Set-up, warm-up, a timed region with per-kernel HIP events, then
-    untimed steps up to `total` for the error report."""
+    """Set-up, warm-up, the timed region (wall clock, no per-kernel events),
+    the same number of steps again with per-kernel HIP events (the launches
+    then run one at a time), then untimed steps up to `total` for the error
+    report."""
     from channel_flow_amd.channel import Channel
     v, oy, au0, w0 = ic_arrays
     t0 = time.perf_counter()
@@ -247,15 +249,16 @@ def run_form(cfg, ic_arrays, precision, steps, warmup, total, log):
     tiles = ch.tile_choices()
     ch.initial(v, oy, au0, w0)
     ch.step(warmup)
-    ch.set_timing(True)
     ch.sync()
     t0 = time.perf_counter()
     ch.step(steps)
     ch.sync()
     wall = time.perf_counter() - t0
+    ch.set_timing(True)
+    ch.step(steps)
     kt = ch.kernel_times()
     ch.set_timing(False)
-    rest = total - warmup - steps
+    rest = total - warmup - 2 * steps
     if rest > 0:
         ch.step(rest)
     st = ch.get_state()
@@ -335,7 +338,7 @@ def main(argv=None):
                              cfg["rms"], cfg["kmax"], cfg["slope"], cfg["seed"])
     log(f"ic {time.perf_counter() - t0:.2f}s")
     forms = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
-    total = max(a.error_steps, a.warmup + a.steps) if a.precision == "both" else a.warmup + a.steps
+    total = max(a.error_steps, a.warmup + 2 * a.steps) if a.precision == "both" else a.warmup + 2 * a.steps
     runs = {f: run_form(cfg, ic_arrays, f, a.steps, a.warmup, total, log) for f in forms}
     head = "mixed" if "mixed" in runs else "fp64"
     r = runs[head]

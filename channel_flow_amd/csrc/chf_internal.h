@@ -82,6 +82,11 @@ struct Ctx {
   Dims d{};
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of the context (default priority): with the omega_y operator
+  // diagonalised, the omega_y right-hand side and solve run on it while the v
+  // path runs on `stream`; the pair of events forks and joins it
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // wavenumbers
   double* alpha = nullptr;  // [Mx]
   double* beta = nullptr;   // [Mz]

@@ -105,8 +105,10 @@ long long chf_device_bytes(void* ctx);
 /* Instrumentation (no counterpart in the reference, whose only timer is the
  * mpi_wtime bracket of the solve loop, PF:217-229, 481-486).  With timing on,
  * every kernel launch of chf_solve/chf_nonlinear/chf_step is bracketed by HIP
- * events on the context stream; chf_timing sums the elapsed time of kernel
- * class `id` (0 <= id < chf_kernel_count()) since the last chf_set_timing. */
+ * events and the launches run one after another (without timing the omega_y
+ * solve overlaps the v solve on a second stream), so each kernel class's time
+ * is its own; chf_timing sums the elapsed time of kernel class `id`
+ * (0 <= id < chf_kernel_count()) since the last chf_set_timing. */
 int chf_set_timing(void* ctx, int on);
 int chf_kernel_count(void);
 const char* chf_kernel_name(int id);

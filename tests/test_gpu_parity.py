@@ -292,3 +292,27 @@ def test_tile_choices_bit_identical(require_gpu, precision):
     for cb in (2, 4):
         for k in STATE_KEYS:
             np.testing.assert_array_equal(states[cb][k], states[1][k], err_msg=f"{precision} cb={cb} {k}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_timing_does_not_change_results(require_gpu, precision):
+    """With per-kernel timing on, the launches of a step run one after another
+    on the context stream (without it the omega_y path overlaps the v path on
+    a second stream): both give bit-identical states, and every kernel class
+    of the step is timed."""
+    cfg = dict(nnx=32, nny=32, nnz=48, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0,
+               dt=5.0e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=26)
+    v, oy, au0, w0 = ic_of(cfg)
+    with make_channel(cfg, precision) as a, make_channel(cfg, precision) as b:
+        a.initial(v, oy, au0, w0)
+        b.initial(v, oy, au0, w0)
+        a.step(3)
+        b.set_timing(True)
+        b.step(3)
+        kt = b.kernel_times()
+        b.set_timing(False)
+        sa, sb = a.get_state(), b.get_state()
+    for k in STATE_KEYS:
+        np.testing.assert_array_equal(sb[k], sa[k], err_msg=f"{precision} {k}")
+    for name in ("rhs_v", "rhs_o", "solve", "wsolve", "kinematics", "synth_padded", "analysis"):
+        assert kt[name][1] == 3, (name, kt[name])
