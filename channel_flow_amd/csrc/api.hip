@@ -595,7 +595,7 @@ int solve(Ctx* c) {
     CHF_TRY(timed(c, K_RHS_PREP, [&] { return launch_rhs_prep(*c); }));
     c->prep_stale = false;
   }
-  if (c->wdiag && !c->timing) {
+  if (c->wdiag && !c->timing && c->overlap) {
     // the omega_y path (right-hand side, diagonalised solve) does not touch
     // the v path's data until the kinematics: it runs on the side stream,
     // the matrix-core-bound omega_y solve beside the HBM-bound v solve.
@@ -1025,6 +1025,7 @@ int chf_ensemble_create(const chf_params* members, int n, void** out) {
   for (int i = 0; i < n; ++i) {
     Ctx* c = new Ctx();
     c->prm = members[i];
+    c->overlap = false;
     const int rc = build(c);
     if (rc) {
       g_create_err = "member " + std::to_string(i) + ": " + c->err;

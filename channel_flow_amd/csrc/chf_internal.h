@@ -87,6 +87,9 @@ struct Ctx {
   // path runs on `stream`; the pair of events forks and joins it
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // off for ensemble members: their steps already run as parallel graph
+  // branches, and the extra fork/join nodes cost more than they overlap
+  bool overlap = true;
   // wavenumbers
   double* alpha = nullptr;  // [Mx]
   double* beta = nullptr;   // [Mz]
