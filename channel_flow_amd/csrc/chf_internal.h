@@ -71,7 +71,9 @@ inline MfmaShape mfma_shape(int M, int K) {
   return s;
 }
 
-// Kernel classes for per-launch timing.
+// Kernel classes for per-launch timing.  K_MEAN_POINTS has no launch of its
+// own: the mean-mode point values come from the last block of the h-assembly
+// launch and are timed with it (the id is kept so the class ids stay stable).
 enum KernelId {
   K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
   K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_KINEMATICS, K_WSOLVE, K_COUNT
