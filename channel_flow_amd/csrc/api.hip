@@ -102,9 +102,9 @@ int validate(const chf_params* p, std::string& msg) {
     return -1;
   }
   if (p->precision != 0 && p->precision != 1) { msg = "precision must be 0 (FP64) or 1 (mixed)"; return -1; }
-  if (p->precision == 1 && (!reg_fft_length(3 * p->nnx / 2) || !reg_fft_length(3 * p->nny / 2) ||
+  if (p->precision == 1 && (!reg_fft_length32(3 * p->nnx / 2) || !reg_fft_length32(3 * p->nny / 2) ||
                             (p->nnx / 2) % 16)) {
-    msg = "mixed precision needs 3*nnx/2 and 3*nny/2 in {12,24,48,96,144,192,384} and nnx % 32 == 0";
+    msg = "mixed precision needs 3*nnx/2 and 3*nny/2 in {12,24,48,96,144,192,384,768} and nnx % 32 == 0";
     return -1;
   }
   return 0;

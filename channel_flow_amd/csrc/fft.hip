@@ -520,13 +520,13 @@ __global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(RegX<V> a) 
 __device__ __forceinline__ int spad(int a) { return a + (a >> 4); }
 
 // One Stockham autosort radix-R stage over a line of L points held in LDS
-// (Ns = product of the radices already applied; thread j < L/R):
+// (Ns = product of the radices already applied; butterfly j < L/R, st_stage
+// runs them on the 64 lanes of a wavefront, two per lane at L/R = 96):
 //   v[r] = in[j + r L/R] * W_{Ns R}^{r (j mod Ns)},  v = DFT_R(v),
 //   out[(j / Ns) Ns R + (j mod Ns) + r Ns] = v[r].
 // After the last stage the line is in natural order.  `tw` = e^{-2 pi i m / L}.
 template <int L, int R, int Ns, int S, class V>
-__device__ __forceinline__ void st_stage(const V* in, V* out, const V* tw, int j) {
-  if (j >= L / R) return;
+__device__ __forceinline__ void st_stage1(const V* in, V* out, const V* tw, int j) {
   V v[R];
   const int k = j % Ns;
 #pragma unroll
@@ -543,6 +543,12 @@ __device__ __forceinline__ void st_stage(const V* in, V* out, const V* tw, int j
   const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
   for (int r = 0; r < R; ++r) out[spad(base + r * Ns)] = v[r];
+}
+
+template <int L, int R, int Ns, int S, class V>
+__device__ __forceinline__ void st_stage(const V* in, V* out, const V* tw, int lane) {
+#pragma unroll
+  for (int j = lane; j < L / R; j += 64) st_stage1<L, R, Ns, S>(in, out, tw, j);
 }
 
 // Orders the LDS accesses of one wavefront: its lanes see each other's earlier
@@ -757,6 +763,11 @@ bool reg_fft_length(int L) {
   }
 }
 
+// Lengths of the FP32 (mixed-form) register kernels: the FP64 ones plus 768
+// (the 512-mode channel), whose FP64 lines would not fit the register file and
+// take the LDS kernels instead.
+bool reg_fft_length32(int L) { return reg_fft_length(L) || L == 768; }
+
 template <int N1, int N2, class V>
 static hipError_t run_xpass_reg(Ctx& c, const RegX<V>& a) {
   using Gm = Geo<N1, N2>;
@@ -782,6 +793,9 @@ static hipError_t xpass_reg(Ctx& c, const RegX<V>& a) {
     case 144: return run_xpass_reg<12, 12>(c, a);
     case 192: return run_xpass_st<192, 8, 8, 3>(c, a);
     case 384: return run_xpass_st<384, 8, 8, 6>(c, a);
+    case 768:  // FP32 only (reg_fft_length32)
+      if constexpr (sizeof(V) == 8) return run_xpass_st<768, 8, 8, 12>(c, a);
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }
@@ -818,6 +832,9 @@ static hipError_t zfft_reg(Ctx& c, const RegZ<V>& a, bool inv, int nf) {
     case 144: return run_zfft_reg<12, 12>(c, a, inv, nf);
     case 192: return run_zfft_reg<12, 16>(c, a, inv, nf);
     case 384: return run_zfft_reg<16, 24>(c, a, inv, nf);
+    case 768:  // FP32 only (reg_fft_length32)
+      if constexpr (sizeof(V) == 8) return run_zfft_reg<24, 32>(c, a, inv, nf);
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }

@@ -228,12 +228,13 @@ def test_mixed_large_degree(require_gpu):
         assert e[k] < 1e-9, (k, e[k])
 
 
-@pytest.mark.parametrize("nnx", [128, 256])
+@pytest.mark.parametrize("nnx", [128, 256, 512])
 def test_long_streamwise_lines(require_gpu, nnx):
-    """Padded streamwise lines of 192 and 384 points (the one-wavefront-per-line
-    Stockham streamwise pass of the 128- and 256-mode workloads) on a short span
-    and low degree: FP64 initial() + 3 steps against the oracle, and the mixed
-    form against the FP64 form."""
+    """Padded streamwise lines of 192, 384 and 768 points (the one-wavefront-per-line
+    Stockham streamwise pass of the 128-, 256- and 512-mode workloads; at 768 a
+    wavefront runs two radix-8 butterflies per lane) on a short span and low
+    degree: FP64 initial() + 3 steps against the oracle, and the mixed form
+    against the FP64 form."""
     cfg = dict(nnx=nnx, nny=16, nnz=12, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0,
                dt=2.5e-4, u_c=1.0, rms=0.1, kmax=48.0, slope=-4.0 / 3.0, seed=23)
     p = params_of(cfg)
@@ -316,3 +317,35 @@ def test_timing_does_not_change_results(require_gpu, precision):
         np.testing.assert_array_equal(sb[k], sa[k], err_msg=f"{precision} {k}")
     for name in ("rhs_v", "rhs_o", "solve", "wsolve", "kinematics", "synth_padded", "analysis"):
         assert kt[name][1] == 3, (name, kt[name])
+
+
+@pytest.mark.parametrize("nny", [256, 512])
+def test_long_span_lines(require_gpu, nny):
+    """Padded span lines of 384 and 768 points (the register span FFTs of the
+    256- and 512-mode workloads, split 16 x 24 and 24 x 32) on a short
+    streamwise extent and low degree: FP64 initial() + 3 steps against the
+    oracle, and the mixed form against the FP64 form."""
+    cfg = dict(nnx=32, nny=nny, nnz=12, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0,
+               dt=2.5e-4, u_c=1.0, rms=0.1, kmax=48.0, slope=-4.0 / 3.0, seed=27)
+    p = params_of(cfg)
+    v, oy, au0, w0 = ic_of(cfg)
+    st = O.initial_presolve_state(v, oy, au0, w0, p)
+    with make_channel(cfg) as ch, make_channel(cfg, "mixed") as mx:
+        ch.initial(v, oy, au0, w0)
+        mx.initial(v, oy, au0, w0)
+        e0 = state_errors(ch.get_state(), st)
+        report(f"nny={nny} initial", e0)
+        O.Stepper(p).step(st, 3)
+        ch.step(3)
+        mx.step(3)
+        g, gm = ch.get_state(), mx.get_state()
+    e1 = state_errors(g, st)
+    report(f"nny={nny} 3 steps", e1)
+    em = state_errors(gm, g)
+    report(f"nny={nny} mixed vs fp64", em)
+    for k, e in e0.items():
+        assert e < ONE_OP_TOL, (k, e)
+    for k, e in e1.items():
+        assert e < TRAJ_TOL, (k, e)
+    for k in ("vnp1", "omegaynp1"):
+        assert em[k] < 1e-5, (k, em[k])
