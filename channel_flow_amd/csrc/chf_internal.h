@@ -184,7 +184,7 @@ extern const char* const kKernelNames[K_COUNT];
 
 // ---- kernel launchers (each returns hipGetLastError()) ----
 bool reg_fft_length(int L);
-bool reg_fft_length32(int L);  // lengths of the FP32 register kernels
+bool reg_fft_length32(int L);  // register span FFTs; FP32 streamwise pass
 hipError_t launch_zfft32(Ctx& c, const float2* const* in, float2* const* out, int nf, bool inv);
 hipError_t launch_xpass32(Ctx& c);
 hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const* out, int nf);

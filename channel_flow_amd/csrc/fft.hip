@@ -763,9 +763,10 @@ bool reg_fft_length(int L) {
   }
 }
 
-// Lengths of the FP32 (mixed-form) register kernels: the FP64 ones plus 768
-// (the 512-mode channel), whose FP64 lines would not fit the register file and
-// take the LDS kernels instead.
+// Lengths of the register span FFTs (both precisions) and of the FP32
+// streamwise pass: the lengths above plus 768 (the 512-mode channel).  The
+// FP64 streamwise lines of 768 points would not fit the register file and
+// take the LDS kernel.
 bool reg_fft_length32(int L) { return reg_fft_length(L) || L == 768; }
 
 template <int N1, int N2, class V>
@@ -832,9 +833,7 @@ static hipError_t zfft_reg(Ctx& c, const RegZ<V>& a, bool inv, int nf) {
     case 144: return run_zfft_reg<12, 12>(c, a, inv, nf);
     case 192: return run_zfft_reg<12, 16>(c, a, inv, nf);
     case 384: return run_zfft_reg<16, 24>(c, a, inv, nf);
-    case 768:  // FP32 only (reg_fft_length32)
-      if constexpr (sizeof(V) == 8) return run_zfft_reg<24, 32>(c, a, inv, nf);
-      return hipErrorInvalidValue;
+    case 768: return run_zfft_reg<24, 32>(c, a, inv, nf);
     default: return hipErrorInvalidValue;
   }
 }
@@ -854,7 +853,7 @@ static RegZ<V> regz_args(Ctx& c, const V* const* in, V* const* out, int nf, bool
   return a;
 }
 
-static bool use_zreg(const Ctx& c) { return reg_fft_length(c.d.Lz) && c.d.Mx % 16 == 0; }
+static bool use_zreg(const Ctx& c) { return reg_fft_length32(c.d.Lz) && c.d.Mx % 16 == 0; }
 
 static hipError_t zfft_reg64(Ctx& c, double2* const* in, double2* const* out, int nf, bool inv) {
   const double2* cin[6];
