@@ -74,7 +74,11 @@ __device__ __forceinline__ T pick6(T const (&a)[6], int z) {
 // column each from the B tiles, in k order whatever the tile shape, and hand
 // it to P::store_centre, so a lone row does not cost a whole 16-row block of
 // matrix-core work.
-template <class P, int RBW, int CB, int NWV>
+//
+// FULL: every wave owns all RBW of its row blocks (MB == NWV * RBW).  The k
+// step then has no row-block guards, and the compiler issues the table reads
+// of all row blocks ahead of their matrix ops.
+template <class P, int RBW, int CB, int NWV, bool FULL>
 __global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
   using TC = typename P::TC;
   using MF = Mf<TC>;
@@ -88,7 +92,9 @@ __global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
   static_assert(BN <= 16 * NWV, "one centre-row column per lane < 16 of each wave");
   extern __shared__ __align__(16) unsigned char stabraw[];
   TC* stab = reinterpret_cast<TC*>(stabraw);  // 4N cosines (table form)
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave index in a scalar register: the row-block guards below are then
+  // uniform branches, not per-lane masks that cut the k step into pieces
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   P p = pb;
   p.select(blockIdx.z);
   const int n4 = p.n4;
@@ -182,8 +188,7 @@ __global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
           for (int c = 0; c < CB; ++c) bv[s][c] = sB[s][kb * 4 + (lane >> 4)][c * 16 + (lane & 15)];
 #pragma unroll
         for (int r = 0; r < RBW; ++r) {
-          const int rb = wave + NWV * r;
-          if (rb < MB) {
+          if (FULL || wave + NWV * r < MB) {
 #pragma unroll
             for (int m = 0; m < NA; ++m) {
               TC a;
@@ -216,7 +221,7 @@ __global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
 #pragma unroll
   for (int r = 0; r < RBW; ++r) {
     const int rb = wave + NWV * r;
-    if (rb >= MB) continue;
+    if (!FULL && rb >= MB) continue;
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       const long col = col0 + c * 16 + (lane & 15);
@@ -238,8 +243,12 @@ template <class P, int RBW, int CB, int NWV>
 hipError_t launch_mfma_cos(hipStream_t st, const P& p, int nz) {
   const long tiles = (p.ncold + 16 * CB - 1) / (16 * CB);
   const size_t shm = cos_recurrence<P>() ? 0 : sizeof(typename P::TC) * p.n4;
-  hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB, NWV>), dim3((unsigned)tiles, 1, nz), dim3(64 * NWV), shm, st,
-                     p);
+  if (p.sh.MB == NWV * RBW)
+    hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB, NWV, true>), dim3((unsigned)tiles, 1, nz), dim3(64 * NWV), shm, st,
+                       p);
+  else
+    hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB, NWV, false>), dim3((unsigned)tiles, 1, nz), dim3(64 * NWV), shm, st,
+                       p);
   return hipGetLastError();
 }
 
