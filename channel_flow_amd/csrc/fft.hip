@@ -756,6 +756,95 @@ __global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_reg(RegZ<V> a) {
   }
 }
 
+// FP32 span pass over column pairs: thread (t, tx) transforms the lines of
+// kx = 2 (kg TX + tx) and kx + 1 together, so every global access is one
+// 16-byte (kx, kx + 1) pair and a row of the block's TX pairs is a whole
+// 128-byte line at TX = 8.  The twiddles are loaded once for both lines, and
+// the LDS exchange moves the pair as one 16-byte value.  Each line sees the
+// same operations as in k_zfft_reg.
+template <int N1, int N2, int S>
+__device__ __forceinline__ void fft_stepA_pair(float2 (&a)[N2], float2 (&b)[N2], float4* buf, const float2* tw,
+                                               int t) {
+  Dft<N2, S, float2>::run(a);
+  Dft<N2, S, float2>::run(b);
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) {
+    float2 w = tw[t * k2];
+    if (S > 0) w.y = -w.y;
+    const float2 va = (k2 == 0) ? a[0] : cmulv(a[k2], w);
+    const float2 vb = (k2 == 0) ? b[0] : cmulv(b[k2], w);
+    buf[t * (N2 + 1) + k2] = make_float4(va.x, va.y, vb.x, vb.y);
+  }
+}
+
+template <int N1, int N2, int S>
+__device__ __forceinline__ void fft_stepB_pair(float2 (&a)[N1], float2 (&b)[N1], const float4* buf, int t) {
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1) {
+    const float4 q = buf[n1 * (N2 + 1) + t];
+    a[n1] = make_float2(q.x, q.y);
+    b[n1] = make_float2(q.z, q.w);
+  }
+  Dft<N1, S, float2>::run(a);
+  Dft<N1, S, float2>::run(b);
+}
+
+template <int N1, int N2, int TX, bool INV>
+__global__ __launch_bounds__((TX * Geo<N1, N2>::G)) void k_zfft_pair(RegZ<float2> a) {
+  using Gm = Geo<N1, N2>;
+  constexpr int L = Gm::L;
+  extern __shared__ __align__(16) unsigned char smraw[];
+  float2* tw = reinterpret_cast<float2*>(smraw);
+  const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
+  float4* buf = reinterpret_cast<float4*>(smraw + sizeof(float2) * L) + tx * Gm::BUF_A;
+  const int Mz = a.Mz, h = Mz / 2;
+  const int kg = (int)xcd_order(blockIdx.x, gridDim.x);
+  const int kp = kg * TX + tx;  // column pair: kx = 2 kp, 2 kp + 1
+  const int Mp = a.Mx / 2;      // pairs per row
+  const int p = blockIdx.y;
+  const float4* in = reinterpret_cast<const float4*>(a.in[blockIdx.z] + (long)p * a.in_plane);
+  float4* out = reinterpret_cast<float4*>(a.out[blockIdx.z] + (long)p * a.out_plane);
+  for (int i = threadIdx.x; i < L; i += TX * Gm::G) tw[i] = a.tw[i];
+  __syncthreads();
+  if (t < N1) {
+    float2 x0[N2], x1[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      const int m = n2 * N1 + t;
+      float4 q = make_float4(0, 0, 0, 0);
+      if (INV) {
+        int kz = -1;
+        if (m < h) kz = m;                       // Nyquist slot Mz/2 stays zero (expand, PF:730)
+        else if (m > L - h) kz = m - L + Mz;
+        if (kz >= 0) q = in[(long)kz * Mp + kp];
+      } else {
+        q = in[(long)m * Mp + kp];
+      }
+      x0[n2] = make_float2(q.x, q.y);
+      x1[n2] = make_float2(q.z, q.w);
+    }
+    fft_stepA_pair<N1, N2, INV ? +1 : -1>(x0, x1, buf, tw, t);
+  }
+  __syncthreads();
+  if (t < N2) {
+    float2 b0[N1], b1[N1];
+    fft_stepB_pair<N1, N2, INV ? +1 : -1>(b0, b1, buf, t);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int m = k1 * N2 + t;
+      const float4 q = make_float4(b0[k1].x, b0[k1].y, b1[k1].x, b1[k1].y);
+      if (INV) {
+        out[(long)m * Mp + kp] = q;
+      } else {
+        int kz = -1;                              // expand(-1): keep 0..Mz/2 and the negative band
+        if (m <= h) kz = m;
+        else if (m > L - h) kz = m - L + Mz;
+        if (kz >= 0) out[(long)kz * Mp + kp] = q;
+      }
+    }
+  }
+}
+
 bool reg_fft_length(int L) {
   switch (L) {
     case 12: case 24: case 48: case 96: case 144: case 192: case 384: return true;
@@ -801,9 +890,44 @@ static hipError_t xpass_reg(Ctx& c, const RegX<V>& a) {
   }
 }
 
+// Column-pair launch (FP32): TX pairs per block, the largest of 8, 4, 2 whose
+// exchange tiles fit in 64 KB of LDS and whose 2 TX columns divide Mx.
+// Returns hipErrorNotSupported when no tile fits (the caller then takes the
+// single-column kernel).
+template <int N1, int N2, int TX>
+static hipError_t launch_zfft_pair(Ctx& c, const RegZ<float2>& a, bool inv, int nf) {
+  using Gm = Geo<N1, N2>;
+  const size_t shm = sizeof(float2) * Gm::L + sizeof(float4) * (size_t)TX * Gm::BUF_A;
+  if (shm > 65536 || c.d.Mx % (2 * TX)) return hipErrorNotSupported;
+  for (int f = 0; f < nf; ++f)
+    if (((uintptr_t)a.in[f] | (uintptr_t)a.out[f]) & 15) return hipErrorNotSupported;
+  dim3 grid(c.d.Mx / (2 * TX), c.d.Py, nf);
+  if (inv)
+    hipLaunchKernelGGL((k_zfft_pair<N1, N2, TX, true>), grid, dim3(TX * Gm::G), shm, c.stream, a);
+  else
+    hipLaunchKernelGGL((k_zfft_pair<N1, N2, TX, false>), grid, dim3(TX * Gm::G), shm, c.stream, a);
+  return hipGetLastError();
+}
+
+template <int N1, int N2>
+static hipError_t run_zfft_pair(Ctx& c, const RegZ<float2>& a, bool inv, int nf) {
+  using Gm = Geo<N1, N2>;
+  constexpr int TX = (256 / Gm::G) >= 8 ? 8 : ((256 / Gm::G) >= 4 ? 4 : 2);
+  hipError_t e = launch_zfft_pair<N1, N2, TX>(c, a, inv, nf);
+  if constexpr (TX > 2)
+    if (e == hipErrorNotSupported) e = launch_zfft_pair<N1, N2, TX / 2>(c, a, inv, nf);
+  if constexpr (TX > 4)
+    if (e == hipErrorNotSupported) e = launch_zfft_pair<N1, N2, TX / 4>(c, a, inv, nf);
+  return e;
+}
+
 template <int N1, int N2, class V>
 static hipError_t run_zfft_reg(Ctx& c, const RegZ<V>& a, bool inv, int nf) {
   using Gm = Geo<N1, N2>;
+  if constexpr (sizeof(V) == 8) {
+    const hipError_t e = run_zfft_pair<N1, N2>(c, a, inv, nf);
+    if (e != hipErrorNotSupported) return e;
+  }
   constexpr int TX = (256 / Gm::G) >= 16 ? 16 : ((256 / Gm::G) >= 8 ? 8 : 4);
   if (c.d.Mx % TX) return hipErrorInvalidValue;
   const size_t shm = sizeof(V) * (Gm::L + (size_t)TX * Gm::LINE_Z);
