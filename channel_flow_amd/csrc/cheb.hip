@@ -31,6 +31,7 @@ struct SynthP {
   static constexpr int NACC = 2, NB = 2, KC = 16;  // KC: k rows per B tile
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = true;
+  static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
   const double* rtab;  // FP64 cosines for the recurrence form
@@ -136,6 +137,7 @@ struct AnalP {
   static constexpr int NACC = 2, NB = 2, KC = 16;  // KC: k rows per B tile
   static constexpr bool kRecurrence = true;
   static constexpr bool kCentre = false;
+  static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
   const double* rtab;  // FP64 cosines for the recurrence form
@@ -310,6 +312,7 @@ struct RhsSynP {
   static constexpr int NACC = 2, NB = 2, KC = 16;  // KC: k rows per B tile
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = false;
+  static constexpr bool kWavePerRowBlock = true;  // 5-6 row blocks: one wave each (measured faster)
   static constexpr int bsel(int m) { return m; }
   const double* costab;
   int n4;

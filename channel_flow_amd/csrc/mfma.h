@@ -243,12 +243,17 @@ template <class P, int RBW, int CB, int NWV>
 hipError_t launch_mfma_cos(hipStream_t st, const P& p, int nz) {
   const long tiles = (p.ncold + 16 * CB - 1) / (16 * CB);
   const size_t shm = cos_recurrence<P>() ? 0 : sizeof(typename P::TC) * p.n4;
-  if (p.sh.MB == NWV * RBW)
+  if (p.sh.MB == NWV * RBW) {
     hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB, NWV, true>), dim3((unsigned)tiles, 1, nz), dim3(64 * NWV), shm, st,
                        p);
-  else
-    hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB, NWV, false>), dim3((unsigned)tiles, 1, nz), dim3(64 * NWV), shm, st,
-                       p);
+  } else {
+    // blocks of more than four waves are launched with one row block per wave only
+    if constexpr (NWV > 4)
+      return hipErrorInvalidValue;
+    else
+      hipLaunchKernelGGL((k_mfma_cos<P, RBW, CB, NWV, false>), dim3((unsigned)tiles, 1, nz), dim3(64 * NWV), shm,
+                         st, p);
+  }
   return hipGetLastError();
 }
 
@@ -271,8 +276,20 @@ hipError_t run_mfma_cos_rbw(hipStream_t st, const P& p, int nz, int cb) {
 
 template <class P>
 hipError_t run_mfma_cos(hipStream_t st, const P& p, int nz, int cb = 0) {
-  // four waves per block (measured: three waves for 6 row blocks, 2 + 2 + 2
-  // instead of 2 + 2 + 1 + 1, is slower: fewer waves hide less latency)
+  // P::kWavePerRowBlock with 5 or 6 row blocks: one wave per row block, so
+  // every wave has the same work (with four waves, 2 + 2 + 1 + 1 leaves two
+  // waves idle for half the k loop).  Measured per problem: faster for the
+  // right-hand-side brackets, slower for the point synthesis; seven-wave
+  // blocks (7 row blocks) were slower for every problem.
+  if constexpr (P::kWavePerRowBlock) {
+    switch (p.sh.MB) {
+      case 5: return run_mfma_cos_rbw<P, 1, 5>(st, p, nz, cb);
+      case 6: return run_mfma_cos_rbw<P, 1, 6>(st, p, nz, cb);
+      default: break;
+    }
+  }
+  // otherwise four waves per block (measured: three waves for 6 row blocks,
+  // 2 + 2 + 2 instead of 2 + 2 + 1 + 1, is slower: fewer waves hide less latency)
   switch ((p.sh.MB + 3) / 4) {
     case 1: return run_mfma_cos_rbw<P, 1, 4>(st, p, nz, cb);
     case 2: return run_mfma_cos_rbw<P, 2, 4>(st, p, nz, cb);
