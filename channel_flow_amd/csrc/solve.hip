@@ -669,11 +669,27 @@ __global__ __launch_bounds__(512) void k_wsolve(WArgs a) {
   const int MB1 = (m + 15) / 16, KB = (m + 3) / 4, MB2 = (n + 15) / 16;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long col0 = (long)blockIdx.x * CT;
-  // stage B: rows r < m, zero rows up to 16 MB1
-  for (int idx = threadIdx.x; idx < 16 * MB1 * CT; idx += blockDim.x) {
-    const int r = idx / CT, cc = idx - r * CT;
-    const long col = col0 + cc;
-    sB[idx] = (r < m && col < a.ncold) ? ro[(long)r * a.ncold + col] : 0.0;
+  // stage B: rows r < m, zero rows up to 16 MB1.  Each thread issues eight
+  // loads before its first LDS store; the loads are unconditional (row and
+  // column clamped into the array, the value zeroed afterwards), so no branch
+  // splits them and they are in flight together.
+  constexpr int SL = 8;
+  const int nst = 16 * MB1 * CT;
+  for (int i0 = threadIdx.x; i0 < nst; i0 += SL * blockDim.x) {
+    double v[SL];
+#pragma unroll
+    for (int u = 0; u < SL; ++u) {
+      const int idx = i0 + u * blockDim.x;
+      const int r = idx / CT, cc = idx - r * CT;
+      const long col = col0 + cc;
+      const double x = ro[(long)min(r, m - 1) * a.ncold + min(col, a.ncold - 1)];
+      v[u] = (idx < nst && r < m && col < a.ncold) ? x : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SL; ++u) {
+      const int idx = i0 + u * blockDim.x;
+      if (idx < nst) sB[idx] = v[u];
+    }
   }
   __syncthreads();
   MF::acc acc[TPW][4];
