@@ -213,7 +213,9 @@ __device__ __forceinline__ double2 stream_load(const double* p) {
 // the mean mode (PF:401-474): U and W with the omega_y-type operator at k^2 = 0.
 // Matrix columns per load batch in the streaming loop of k_solve; two batches
 // alternate, so the next batch is in flight while the current one is used.
-constexpr int kSolveSU = 4;
+// Measured at 256x193x256: 6 (12 loads in flight per lane, 2 waves per SIMD)
+// beats 4 (8 loads, 3 waves), 5 and 8.
+constexpr int kSolveSU = 6;
 
 __global__ __launch_bounds__(512) void k_solve(SolveArgs a) {
   extern __shared__ double rb[];  // [slot][Ny][4 vec], slot = group * neq + eq
