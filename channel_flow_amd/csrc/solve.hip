@@ -646,8 +646,11 @@ hipError_t launch_solve(Ctx& c) {
 // are staged in LDS, Y = L B runs on v_mfma_f64_16x16x4f64 with the A
 // fragments of L read from L2, Y is scaled per (row, mode) and written back
 // to LDS as the B operand of X = R Y, and X lands in omega_y at degrees 2i + p.
-// Eight wavefronts per block, each owning up to two 16-row blocks, keep enough
-// independent fragment loads in flight to cover the L2 latency.
+// Eight wavefronts per block own the 16-row blocks, TPW each (wave w: row
+// blocks w, w + 8, ...).  TPW = 1 whenever at most 8 row blocks exist (N up to
+// 254): the kernel then needs 70 VGPRs, so three blocks fit per CU and its
+// waves also fit beside the v solve's (measured 0.164 -> 0.143 ms against
+// TPW = 2, which larger N take).
 struct WArgs {
   const double *L0, *L1, *R0, *R1, *lam0, *lam1;
   const double *ro0, *ro1;  // parity-folded right-hand sides, real view [rows][2 ncol]
@@ -658,9 +661,10 @@ struct WArgs {
   double c;
 };
 
+template <int TPW>
 __global__ __launch_bounds__(512) void k_wsolve(WArgs a) {
   using MF = Mf<double>;
-  constexpr int CT = 64, NW = 8, TPW = 2;  // columns, waves, row blocks per wave (<= 16 row blocks)
+  constexpr int CT = 64, NW = 8;  // columns, waves (<= 8 TPW row blocks)
   extern __shared__ double sB[];   // [rows][CT]
   const int par = blockIdx.y;
   const int n = par ? a.n1 : a.n0, m = n - 1;
@@ -814,9 +818,13 @@ hipError_t launch_wsolve(Ctx& c) {
   const int m = d.ne - 1;
   const size_t shm = sizeof(double) * 64 * 16 * ((m + 15) / 16);
   if (shm > 65536) return hipErrorInvalidValue;
-  if ((d.ne + 15) / 16 > 16) return hipErrorInvalidValue;
+  const int mb = (d.ne + 15) / 16;  // row blocks of the larger (even) half system
+  if (mb > 16) return hipErrorInvalidValue;
   const unsigned tiles = (unsigned)((a.ncold + 63) / 64);
-  hipLaunchKernelGGL(k_wsolve, dim3(tiles, 2), dim3(512), shm, c.stream, a);
+  if (mb <= 8)
+    hipLaunchKernelGGL(k_wsolve<1>, dim3(tiles, 2), dim3(512), shm, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_wsolve<2>, dim3(tiles, 2), dim3(512), shm, c.stream, a);
   return hipGetLastError();
 }
 

@@ -185,10 +185,12 @@ def test_mixed_form_against_fp64_and_oracle(require_gpu):
         assert e1[k] < 1e-10, (k, e1[k])
 
 
-@pytest.mark.parametrize("nnz", [96, 192])
+@pytest.mark.parametrize("nnz", [96, 192, 256])
 def test_large_degree_small_box(require_gpu, nnz):
     """N = 96 and 192 (the multi-row-block contraction paths used at the
-    256x193x256 workload) on a 16x16 box: initial() + 3 steps vs the oracle."""
+    256x193x256 workload) and 256 (the 512x257x512 degree, whose diagonalised
+    omega_y solve takes two row blocks per wave) on a 16x16 box: initial() + 3
+    steps vs the oracle."""
     cfg = dict(nnx=16, nny=16, nnz=nnz, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
                dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=21)
     p = params_of(cfg)
