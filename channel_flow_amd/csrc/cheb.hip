@@ -32,6 +32,7 @@ struct SynthP {
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = true;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
+  static constexpr int kMinWaves = 5;  // measured: 0.373 -> 0.366 ms FP32, 0.648 -> 0.639 FP64
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
   const double* rtab;  // FP64 cosines for the recurrence form
@@ -138,6 +139,7 @@ struct AnalP {
   static constexpr bool kRecurrence = true;
   static constexpr bool kCentre = false;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
+  static constexpr int kMinWaves = 1;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
   const double* rtab;  // FP64 cosines for the recurrence form
@@ -313,6 +315,7 @@ struct RhsSynP {
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = false;
   static constexpr bool kWavePerRowBlock = true;  // 5-6 row blocks: one wave each (measured faster)
+  static constexpr int kMinWaves = 1;
   static constexpr int bsel(int m) { return m; }
   const double* costab;
   int n4;

@@ -78,8 +78,17 @@ __device__ __forceinline__ T pick6(T const (&a)[6], int z) {
 // FULL: every wave owns all RBW of its row blocks (MB == NWV * RBW).  The k
 // step then has no row-block guards, and the compiler issues the table reads
 // of all row blocks ahead of their matrix ops.
+// Minimum waves per SIMD the compiler must leave room for: P::kMinWaves for
+// tiles whose accumulators take at most 48 registers (wider tiles would
+// spill), otherwise no constraint.
+template <class P, int RBW, int CB>
+constexpr int min_waves() {
+  return (P::NACC * RBW * CB * 4 * (int)sizeof(typename P::TC) / 4 <= 48) ? P::kMinWaves : 1;
+}
+
 template <class P, int RBW, int CB, int NWV, bool FULL>
-__global__ __launch_bounds__(64 * NWV) void k_mfma_cos(P pb) {
+__global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(min_waves<P, RBW, CB>())))
+void k_mfma_cos(P pb) {
   using TC = typename P::TC;
   using MF = Mf<TC>;
   constexpr int NT = 64 * NWV;
