@@ -32,7 +32,8 @@ struct SynthP {
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = true;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
-  static constexpr int kMinWaves = 5;  // measured: 0.373 -> 0.366 ms FP32, 0.648 -> 0.639 FP64
+  // measured: FP32 five waves 0.372 -> 0.366 ms; FP64 six waves 0.640 -> 0.625 ms
+  static constexpr int kMinWaves = sizeof(TCc) == 8 ? 6 : 5;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
   const double* rtab;  // FP64 cosines for the recurrence form
