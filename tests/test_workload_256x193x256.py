@@ -1,0 +1,149 @@
+"""The HIP timestep at the full 256x193x256 workload (Re_tau 395 class).
+
+The other GPU tests hold the path to the oracle on boxes small enough for
+the oracle to step many times.  Here the same kernels run at the timed
+workload's own shapes (384x385x384 padded grid, 10368 operator classes):
+
+* initial() (kinematics of every mode + one getnonlinear, PF:1932-1946,
+  PF:1380-1612) against the oracle on the full field, FP64, <= 1e-11;
+* one solve (PF:242-474) against the oracle on 512 sampled (kx, kz) modes
+  plus the mean mode, FP64, <= 1e-11;
+* size-independent properties after 5 steps in both precisions: the
+  continuity iαu + Dv + iβw = 0 and ω_y = iβu − iαw of every solved mode,
+  the Hermitian kx = 0 plane, the unsolved spanwise Nyquist column and the
+  frozen v(0,0), ω_y(0,0) (PF:247, 2365); the mixed form against the FP64
+  form at the bounds of the small-box mixed test.
+
+The oracle's initial state at this size takes tens of seconds of numpy on
+the host and about 12 GB of host memory; it is computed once per module.
+"""
+import numpy as np
+import pytest
+
+from channel_flow_amd import ic
+from oracle import cf_oracle as O
+
+from .helpers import ic_of, params_of, rel_err, state_errors
+
+pytestmark = pytest.mark.gpu
+
+ONE_OP_TOL = 1e-11
+CFG = ic.CONFIGS["retau395_256x193x256"]
+
+
+def make_channel(precision):
+    from channel_flow_amd.channel import Channel
+    c = CFG
+    return Channel(c["nnx"], c["nny"], c["nnz"], c["aphi"], c["beta"], c["dt"], c["re"], c["re_p"],
+                   precision=precision)
+
+
+def report(tag, errs):
+    print(tag, " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+
+
+@pytest.fixture(scope="module")
+def full_case(require_gpu):
+    p = params_of(CFG)
+    arrays = ic_of(CFG)
+    print("oracle initial state at 256x193x256", flush=True)
+    ref = O.initial_presolve_state(*arrays, p)
+    return p, arrays, ref
+
+
+def test_full_size_initial_matches_oracle(full_case):
+    p, arrays, ref = full_case
+    with make_channel("fp64") as ch:
+        ch.initial(*arrays)
+        got = ch.get_state()
+    errs = state_errors(got, ref)
+    report("256x193x256 initial", errs)
+    for k, e in errs.items():
+        assert e < ONE_OP_TOL, (k, e)
+
+
+def test_full_size_solve_on_sampled_modes(full_case):
+    p, _, ref = full_case
+    stp = O.Stepper(p)
+    rng = np.random.default_rng(395)
+    pick = rng.choice(len(stp.solve_modes), size=512, replace=False)
+    stp.solve_modes = [stp.solve_modes[i] for i in sorted(pick)]
+    st = O.copy_state(ref)
+    stp.solve(st)                   # the sampled modes and the mean mode
+    with make_channel("fp64") as ch:
+        ch.set_state(ref)
+        ch.solve()
+        got = ch.get_state()
+        fl = ch.fields()
+    got.update(fl)
+    ii = np.array([m[0] for m in stp.solve_modes])
+    jj = np.array([m[1] for m in stp.solve_modes])
+    errs = {}
+    for k in ("vnp1", "omegaynp1", "unp1", "wnp1", "omegaxnp1", "omegaznp1"):
+        errs[k] = rel_err(got[k][:, jj, ii], st[k][:, jj, ii])
+    scale = max(float(np.max(np.abs(st["au0"]))), float(np.max(np.abs(st["w0"]))))
+    for k in ("au0", "w0"):
+        errs[k] = float(np.max(np.abs(got[k] - st[k]))) / scale
+    report("256x193x256 solve, 512 modes + mean", errs)
+    for k, e in errs.items():
+        assert e < ONE_OP_TOL, (k, e)
+
+
+def _invariants(p, st, fl):
+    """Relative residuals of the properties every solved state has."""
+    alpha, beta = O.wavenumbers(p)
+    a3 = alpha[None, None, :]
+    b3 = beta[None, :, None]
+    v, oy, u, w = st["vnp1"], st["omegaynp1"], fl["unp1"], fl["wnp1"]
+    mask = np.ones((1, p.Mz, p.Mx), dtype=bool)
+    mask[0, p.Mz // 2, :] = False
+    mask[0, 0, 0] = False
+    dv = O.dz(v)
+    vs = max(float(np.max(np.abs(dv))), 1e-300)
+    us = max(float(np.max(np.abs(a3 * u))), float(np.max(np.abs(b3 * w))))
+    div = np.where(mask, 1j * a3 * u + dv + 1j * b3 * w, 0.0)
+    curl = np.where(mask, 1j * b3 * u - 1j * a3 * w - oy, 0.0)
+    mz = p.Mz
+    jr = (mz - np.arange(1, mz // 2)) % mz
+    herm = max(float(np.max(np.abs(f[:, jr, 0] - np.conj(f[:, 1:mz // 2, 0]))))
+               / float(np.max(np.abs(f))) for f in (v, oy))
+    return {
+        "divergence": float(np.max(np.abs(div))) / max(vs, us),
+        "omega_y_curl": float(np.max(np.abs(curl))) / float(np.max(np.abs(oy))),
+        "hermitian_kx0": herm,
+        "nyquist": max(float(np.max(np.abs(v[:, mz // 2, :]))),
+                       float(np.max(np.abs(oy[:, mz // 2, :])))),
+        "frozen_00": max(float(np.max(np.abs(v[:, 0, 0]))), float(np.max(np.abs(oy[:, 0, 0])))),
+    }
+
+
+def test_full_size_invariants_and_mixed_form(full_case):
+    """The kinematics and the solves are FP64 in both forms, so continuity
+    and the ω_y identity hold to FP64 rounding in both.  The Hermitian
+    symmetry of the kx = 0 plane comes from the real physical products:
+    FP64 rounding in the FP64 form, FP32 rounding of the products scaled by
+    dt in the mixed form (bounded by its 1e-5 distance from the FP64 form)."""
+    p, arrays, _ = full_case
+    states = {}
+    herm_tol = {"fp64": 1e-12, "mixed": 1e-5}
+    for precision in ("fp64", "mixed"):
+        with make_channel(precision) as ch:
+            ch.initial(*arrays)
+            ch.step(5)
+            st = ch.get_state()
+            fl = ch.fields()
+        inv = _invariants(p, st, fl)
+        report(f"256x193x256 {precision} 5 steps", inv)
+        assert all(np.all(np.isfinite(st[k])) for k in ("vnp1", "omegaynp1", "au0"))
+        assert inv["divergence"] < 1e-12, inv
+        assert inv["omega_y_curl"] < 1e-12, inv
+        assert inv["hermitian_kx0"] < herm_tol[precision], inv
+        assert inv["nyquist"] == 0.0, inv
+        assert inv["frozen_00"] == 0.0, inv
+        states[precision] = st
+    e = state_errors(states["mixed"], states["fp64"])
+    report("256x193x256 mixed vs fp64, 5 steps", e)
+    for k in ("vnp1", "omegaynp1"):
+        assert e[k] < 1e-5, (k, e[k])
+    for k in ("au0", "w0"):
+        assert e[k] < 1e-9, (k, e[k])
