@@ -63,3 +63,35 @@ def state_errors(gpu, ref):
 
 SMALL = dict(nnx=16, nny=16, nnz=16, aphi=1.0, beta=0.5, re=500.0, re_p=2.0 / 500.0, dt=1.0e-3,
              u_c=1.0, rms=5.0e-2, kmax=4.0, slope=0.0, seed=7)
+
+
+def solved_state_invariants(p, st, fl):
+    """Relative residuals of the properties every solved state has: continuity
+    iαu + Dv + iβw = 0 and ω_y = iβu − iαw for k² ≠ 0 (PF:2357-2403), the
+    Hermitian kx = 0 plane, the unsolved spanwise Nyquist column and the
+    frozen v(0,0), ω_y(0,0) (PF:247, 2365).  `st` holds v, ω_y; `fl` holds
+    the post-solve u, w."""
+    alpha, beta = O.wavenumbers(p)
+    a3 = alpha[None, None, :]
+    b3 = beta[None, :, None]
+    v, oy, u, w = st["vnp1"], st["omegaynp1"], fl["unp1"], fl["wnp1"]
+    mask = np.ones((1, p.Mz, p.Mx), dtype=bool)
+    mask[0, p.Mz // 2, :] = False
+    mask[0, 0, 0] = False
+    dv = O.dz(v)
+    vs = max(float(np.max(np.abs(dv))), 1e-300)
+    us = max(float(np.max(np.abs(a3 * u))), float(np.max(np.abs(b3 * w))))
+    div = np.where(mask, 1j * a3 * u + dv + 1j * b3 * w, 0.0)
+    curl = np.where(mask, 1j * b3 * u - 1j * a3 * w - oy, 0.0)
+    mz = p.Mz
+    jr = (mz - np.arange(1, mz // 2)) % mz
+    herm = max(float(np.max(np.abs(f[:, jr, 0] - np.conj(f[:, 1:mz // 2, 0]))))
+               / float(np.max(np.abs(f))) for f in (v, oy))
+    return {
+        "divergence": float(np.max(np.abs(div))) / max(vs, us),
+        "omega_y_curl": float(np.max(np.abs(curl))) / float(np.max(np.abs(oy))),
+        "hermitian_kx0": herm,
+        "nyquist": max(float(np.max(np.abs(v[:, mz // 2, :]))),
+                       float(np.max(np.abs(oy[:, mz // 2, :])))),
+        "frozen_00": max(float(np.max(np.abs(v[:, 0, 0]))), float(np.max(np.abs(oy[:, 0, 0])))),
+    }

@@ -405,3 +405,35 @@ def test_plane_statistics_single_modes():
     np.testing.assert_allclose(s[7], 0, **tol)
     np.testing.assert_allclose(s[8], 0, **tol)
     np.testing.assert_allclose(s[9], np.abs(U) + np.abs(A), **tol)
+
+
+# ------------------------------------------- invariants of a solved state
+
+
+def test_solved_state_invariants_hold_and_catch_planted_mistakes():
+    """The properties the full-size GPU tests check (continuity, the ω_y
+    identity, the Hermitian kx = 0 plane, the Nyquist column and the frozen
+    (0,0) modes) hold to rounding on the oracle's own state after one solve,
+    and each mistake planted in that state trips the property it breaks."""
+    from channel_flow_amd import ic
+    from .helpers import ic_of, params_of, solved_state_invariants
+
+    cfg = ic.CONFIGS["re3000_32x33x32"]
+    p = params_of(cfg)
+    st = O.initial_presolve_state(*ic_of(cfg), p)
+    O.Stepper(p).solve(st)
+    inv = solved_state_invariants(p, st, st)
+    for k in ("divergence", "omega_y_curl", "hermitian_kx0"):
+        assert inv[k] < 1e-14, (k, inv[k])
+    assert inv["nyquist"] == 0.0 and inv["frozen_00"] == 0.0
+
+    def planted(key, idx, rel=1e-6):
+        bad = O.copy_state(st)
+        bad[key][idx] += rel * np.max(np.abs(bad[key]))
+        return solved_state_invariants(p, bad, bad)
+
+    assert planted("unp1", (5, 3, 2))["divergence"] > 1e-8
+    assert planted("wnp1", (5, 3, 2))["omega_y_curl"] > 1e-8
+    assert planted("vnp1", (4, 3, 0))["hermitian_kx0"] > 1e-8
+    assert planted("omegaynp1", (4, p.Mz // 2, 1))["nyquist"] > 0.0
+    assert planted("vnp1", (4, 0, 0))["frozen_00"] > 0.0

@@ -27,7 +27,7 @@ import pytest
 from channel_flow_amd import ic
 from oracle import cf_oracle as O
 
-from .helpers import ic_of, params_of, rel_err, state_errors
+from .helpers import ic_of, params_of, rel_err, solved_state_invariants, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -95,34 +95,6 @@ def test_solve_on_sampled_modes(oracle_case):
         assert e < ONE_OP_TOL, (k, e)
 
 
-def _invariants(p, st, fl):
-    """Relative residuals of the properties every solved state has."""
-    alpha, beta = O.wavenumbers(p)
-    a3 = alpha[None, None, :]
-    b3 = beta[None, :, None]
-    v, oy, u, w = st["vnp1"], st["omegaynp1"], fl["unp1"], fl["wnp1"]
-    mask = np.ones((1, p.Mz, p.Mx), dtype=bool)
-    mask[0, p.Mz // 2, :] = False
-    mask[0, 0, 0] = False
-    dv = O.dz(v)
-    vs = max(float(np.max(np.abs(dv))), 1e-300)
-    us = max(float(np.max(np.abs(a3 * u))), float(np.max(np.abs(b3 * w))))
-    div = np.where(mask, 1j * a3 * u + dv + 1j * b3 * w, 0.0)
-    curl = np.where(mask, 1j * b3 * u - 1j * a3 * w - oy, 0.0)
-    mz = p.Mz
-    jr = (mz - np.arange(1, mz // 2)) % mz
-    herm = max(float(np.max(np.abs(f[:, jr, 0] - np.conj(f[:, 1:mz // 2, 0]))))
-               / float(np.max(np.abs(f))) for f in (v, oy))
-    return {
-        "divergence": float(np.max(np.abs(div))) / max(vs, us),
-        "omega_y_curl": float(np.max(np.abs(curl))) / float(np.max(np.abs(oy))),
-        "hermitian_kx0": herm,
-        "nyquist": max(float(np.max(np.abs(v[:, mz // 2, :]))),
-                       float(np.max(np.abs(oy[:, mz // 2, :])))),
-        "frozen_00": max(float(np.max(np.abs(v[:, 0, 0]))), float(np.max(np.abs(oy[:, 0, 0])))),
-    }
-
-
 @pytest.mark.parametrize("name", ALL_SIZES)
 def test_invariants_and_mixed_form(require_gpu, name):
     """The kinematics and the solves are FP64 in both forms, so continuity
@@ -141,7 +113,7 @@ def test_invariants_and_mixed_form(require_gpu, name):
             ch.step(5)
             st = ch.get_state()
             fl = ch.fields()
-        inv = _invariants(p, st, fl)
+        inv = solved_state_invariants(p, st, fl)
         report(f"{name} {precision} 5 steps", inv)
         assert all(np.all(np.isfinite(st[k])) for k in ("vnp1", "omegaynp1", "au0"))
         assert inv["divergence"] < 1e-12, inv
