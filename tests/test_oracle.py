@@ -12,6 +12,8 @@ import pytest
 
 from oracle import cf_oracle as O
 
+from . import oracle_pins
+
 
 def small_params(nnx=8, nny=8, nnz=6, aphi=1.0, beta=0.5, dt=1e-3, re=100.0, re_p=0.0):
     return O.Params(nnx=nnx, nny=nny, nnz=nnz, aphi=aphi, beta=beta, dt=dt, re=re, re_p=re_p)
@@ -217,44 +219,14 @@ def test_laminar_profile_is_a_fixed_point():
         assert np.max(np.abs(st[k])) < 1e-12, k
 
 
-def collocation_eig(p, kmn, which):
-    """Generalised eigenpairs of the CN pair (A, B) on the solved subspace."""
-    ops = O.Operators(p)
-    if which == "omegay":
-        a, brows, nb = ops.coef_omegay(kmn), ops.rhs_op_omegay(kmn), 2
-    else:
-        a, brows, nb = ops.coef_v(kmn), ops.rhs_op_v(kmn), 4
-    b = np.zeros_like(a)
-    b[nb:] = brows
-    # B x = g A x restricted to vectors satisfying the BC rows: eig of A^-1 B
-    g, vec = np.linalg.eig(np.linalg.inv(a) @ b)
-    return g, vec
-
-
-@pytest.mark.parametrize("which", ["omegay", "v"])
-def test_crank_nicolson_amplification(which):
-    """An eigenvector of A^-1 B is multiplied by its eigenvalue in one step,
-    and that eigenvalue is the CN factor (1 + c lam)/(1 - c lam) of the
-    collocation operator's eigenvalue lam (c = dt/2re)."""
-    p = small_params(nnx=8, nny=8, nnz=20, re=50.0, dt=2e-3)
-    kmn = 1.0 + 4.0   # mode (i=1, j=1): alpha=1, beta=2
-    g, vec = collocation_eig(p, kmn, which)
-    order = np.argsort(-np.abs(g))
-    g = g[order]
-    vec = vec[:, order]
-    # pick the least damped non-trivial eigenpair
-    k = np.argmax(np.abs(g) < 1.0 - 1e-12)
-    mode = vec[:, k] * 1e-9
-    st = O.zero_state(p)
-    key = "omegaynp1" if which == "omegay" else "vnp1"
-    st[key][:, 1, 1] = mode
-    stepper = O.Stepper(p)
-    stepper.solve(st)
-    np.testing.assert_allclose(st[key][:, 1, 1], g[k] * mode, rtol=1e-9, atol=1e-20)
-    # the CN factor relation: g = (1 + c lam)/(1 - c lam)
-    c = p.dt * 0.5 / p.re
-    lam = (g[k] - 1.0) / (c * (g[k] + 1.0))
-    assert lam.real < 0
+@pytest.mark.parametrize("pin", oracle_pins.ALL_PINS, ids=[f.__name__ for f in oracle_pins.ALL_PINS])
+def test_physics_pin(pin):
+    """The physics pins of tests/oracle_pins.py: the linearised nonlinear term
+    is the Orr-Sommerfeld/Squire operator, a solve is the Crank-Nicolson/AB2
+    step of operators assembled from numpy derivatives, Dirichlet eigenmodes
+    decay by the Crank-Nicolson factor, the mean Lamb vector is the
+    Reynolds-stress gradient, and solved modes carry the curl of the velocity."""
+    pin(O)
 
 
 def test_kinematic_identities_after_solve():
@@ -423,9 +395,9 @@ def test_solved_state_invariants_hold_and_catch_planted_mistakes():
     st = O.initial_presolve_state(*ic_of(cfg), p)
     O.Stepper(p).solve(st)
     inv = solved_state_invariants(p, st, st)
-    for k in ("divergence", "omega_y_curl", "hermitian_kx0"):
+    for k in ("divergence", "omega_y_curl", "hermitian_kx0", "mean_curl"):
         assert inv[k] < 1e-14, (k, inv[k])
-    assert inv["nyquist"] == 0.0 and inv["frozen_00"] == 0.0
+    assert inv["nyquist"] == 0.0 and inv["frozen_00"] == 0.0 and inv["mean_velocity"] == 0.0
 
     def planted(key, idx, rel=1e-6):
         bad = O.copy_state(st)
@@ -437,3 +409,14 @@ def test_solved_state_invariants_hold_and_catch_planted_mistakes():
     assert planted("vnp1", (4, 3, 0))["hermitian_kx0"] > 1e-8
     assert planted("omegaynp1", (4, p.Mz // 2, 1))["nyquist"] > 0.0
     assert planted("vnp1", (4, 0, 0))["frozen_00"] > 0.0
+    for key, idx in (("omegaxnp1", (3, 0, 0)), ("omegaznp1", (2, 0, 0))):
+        bad = O.copy_state(st)
+        bad[key][idx] += 1e-6                              # the mean shear is O(1)
+        assert solved_state_invariants(p, bad, bad)["mean_curl"] > 1e-8, key
+    # measured against the starting state, a frozen entry that was overwritten shows
+    before = O.copy_state(st)
+    for key, idx in (("vnp1", (3, 0, 0)), ("omegaynp1", (2, p.Mz // 2, 3))):
+        bad = O.copy_state(st)
+        bad[key][idx] = 0.0 if bad[key][idx] != 0 else 1.0
+        inv = solved_state_invariants(p, bad, bad, before=before)
+        assert inv["frozen_00"] > 0.0 or inv["nyquist"] > 0.0, (key, inv)
