@@ -28,7 +28,7 @@ class Channel:
         rc = self._lib.chf_create(ctypes.byref(self.params), ctypes.byref(self._ctx))
         if rc != 0:
             msg = self._lib.chf_last_error(None).decode()
-            raise rt.ChfError(f"chf_create failed ({rc}): {msg}")
+            raise rt.error(f"chf_create failed ({rc}): {msg}", rc)
         self._owned = True
         self._init_dims(precision)
 
@@ -53,7 +53,7 @@ class Channel:
     def _check(self, rc, what):
         if rc != 0:
             msg = self._lib.chf_last_error(self._ctx).decode()
-            raise rt.ChfError(f"{what} failed ({rc}): {msg}")
+            raise rt.error(f"{what} failed ({rc}): {msg}", rc)
 
     def close(self):
         if self._ctx and self._owned:

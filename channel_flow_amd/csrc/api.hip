@@ -2,6 +2,7 @@
 // layout, and the step sequence of the reference time loop (PF:220-532).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -130,7 +131,7 @@ void free_all(Ctx* c) {
   f(c->inv); f(c->inv0); f(c->crep); f(c->bmat); f(c->bg0); f(c->bg1);
   for (int i = 0; i < 2; ++i) { f(c->wL[i]); f(c->wR[i]); f(c->wlam[i]); }
   f(c->phys_c); f(c->phys_r[0]); f(c->phys_r[1]); f(c->phys_r[2]); f(c->stats); f(c->twxu); f(c->twzu); f(c->twx); f(c->twz); f(c->costab);
-  f(c->gv); f(c->go);
+  f(c->gv); f(c->go); f(c->nonfinite);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -383,6 +384,7 @@ int build(Ctx* c) {
   CHF_TRY(dalloc(c, &c->roO, (size_t)d.ne * d.ncol));
   CHF_TRY(dalloc(c, &c->gv, fld));
   CHF_TRY(dalloc(c, &c->go, fld));
+  CHF_TRY(dalloc(c, &c->nonfinite, 1));
 
   // wavenumbers (PF:123-134, id = 0)
   std::vector<double> al(d.Mx), be(d.Mz);
@@ -442,9 +444,12 @@ int build(Ctx* c) {
   CHF_TRY(hipMemcpy(c->t4, t4.data(), tt * sizeof(double), hipMemcpyHostToDevice));
 
   // omega_y operator: diagonalised when the decomposition verifies, else its
-  // half-system inverses are stored next to the v ones
+  // half-system inverses are stored next to the v ones.  CHF_OMEGA_INVERSES=stored
+  // in the environment selects the stored inverses without trying.
   {
-    const int rc = omega_diagonalise(c, t0, t2, al, be);
+    const char* force = std::getenv("CHF_OMEGA_INVERSES");
+    const bool stored = force && std::strcmp(force, "stored") == 0;
+    const int rc = stored ? 0 : omega_diagonalise(c, t0, t2, al, be);
     if (rc) return rc;
     const long neP = d.ne + (d.ne & 1), noP = d.no + (d.no & 1);
     if (c->wdiag) {
@@ -528,6 +533,7 @@ int build(Ctx* c) {
   for (double2* q : {c->cl1[0], c->cl1[1], c->cl3[0], c->cl3[1], c->au0, c->w0})
     CHF_TRY(hipMemsetAsync(q, 0, d.Ny * sizeof(double2), c->stream));
   CHF_TRY(hipMemsetAsync(c->vst, 0, (size_t)d.ngroups * d.Ny * 2 * sizeof(double2), c->stream));
+  CHF_TRY(hipMemsetAsync(c->nonfinite, 0, sizeof(int), c->stream));
 
   // kernels that stage more than 64 KiB of LDS
   CHF_TRY(launch_setup_inverses(*c));
@@ -591,10 +597,6 @@ int nonlinear(Ctx* c) {
 }
 
 int solve(Ctx* c) {
-  if (c->prep_stale) {
-    CHF_TRY(timed(c, K_RHS_PREP, [&] { return launch_rhs_prep(*c); }));
-    c->prep_stale = false;
-  }
   if (c->wdiag && !c->timing && c->overlap) {
     // the omega_y path (right-hand side, diagonalised solve) does not touch
     // the v path's data until the kinematics: it runs on the side stream,
@@ -620,6 +622,28 @@ int solve(Ctx* c) {
   }
   CHF_TRY(timed(c, K_KINEMATICS, [&] { return launch_kinematics(*c); }));
   return 0;
+}
+
+// Waits for the context's work and reports a state that the last kinematics
+// pass found to hold a NaN or an infinity (v, omega_y, U or W).
+int check_finite(Ctx* c, const char* who) {
+  int flag = 0;
+  CHF_TRY(hipMemcpyAsync(&flag, c->nonfinite, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  CHF_TRY(hipStreamSynchronize(c->stream));
+  if (flag) {
+    c->err = std::string(who) + ": the state is not finite (NaN or infinity in v, omega_y, U or W)";
+    return CHF_NONFINITE;
+  }
+  return 0;
+}
+
+// A new v, omega_y, U, W: clear the non-finite flag, then derive u, w,
+// omega_x, omega_z and the next solve's right-hand-side brackets with the
+// kinematics pass that follows every solve.
+hipError_t load_kinematics(Ctx* c) {
+  hipError_t e = hipMemsetAsync(c->nonfinite, 0, sizeof(int), c->stream);
+  if (e != hipSuccess) return e;
+  return launch_kinematics(*c);
 }
 
 }  // namespace
@@ -677,9 +701,9 @@ int chf_set_state(void* ctx, const chf_state* st) {
   for (auto& e : m)
     if (!e.src) return fail(c, "chf_set_state: null array");
   for (auto& e : m) CHF_TRY(hipMemcpyAsync(e.dst, e.src, e.bytes, hipMemcpyHostToDevice, c->stream));
-  CHF_TRY(hipStreamSynchronize(c->stream));
-  c->prep_stale = true;  // the RHS brackets of the old v, omega_y no longer apply
-  return 0;
+  // u, w, omega_x, omega_z and the brackets of the loaded v, omega_y, U, W
+  CHF_TRY(load_kinematics(c));
+  return check_finite(c, "chf_set_state");
 }
 
 int chf_get_state(void* ctx, const chf_state* st) {
@@ -720,12 +744,10 @@ int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1, cons
     CHF_TRY(hipMemsetAsync(q, 0, mb, c->stream));
   // fields with all-zero u, w, omega_x, omega_z at the Nyquist column
   for (double2* q : {c->u, c->w, c->ox, c->oz, c->gv, c->go}) CHF_TRY(hipMemsetAsync(q, 0, fb, c->stream));
-  CHF_TRY(launch_kinematics(*c));
-  c->prep_stale = false;  // the kinematics pass wrote the brackets
+  CHF_TRY(load_kinematics(c));
   int rc = nonlinear(c);
   if (rc) return rc;
-  CHF_TRY(hipStreamSynchronize(c->stream));
-  return 0;
+  return check_finite(c, "chf_init_fields");
 }
 
 int chf_nonlinear(void* ctx) {
@@ -735,7 +757,9 @@ int chf_nonlinear(void* ctx) {
 
 int chf_solve(void* ctx) {
   if (!ctx) return -1;
-  return solve(static_cast<Ctx*>(ctx));
+  Ctx* c = static_cast<Ctx*>(ctx);
+  const int rc = solve(c);
+  return rc ? rc : check_finite(c, "chf_solve");
 }
 
 int chf_step(void* ctx, int n) {
@@ -748,7 +772,7 @@ int chf_step(void* ctx, int n) {
     rc = nonlinear(c);
     if (rc) return rc;
   }
-  return 0;
+  return check_finite(c, "chf_step");
 }
 
 int chf_sync(void* ctx) {
@@ -845,7 +869,7 @@ int chf_physical(void* ctx, int field, double* out) {
   if (!out) return fail(c, "chf_physical: null output");
   const int rc = ensure_physical(c);
   if (rc) return rc;
-  CHF_TRY(launch_physical(*c, f, c->phys_r[0]));
+  CHF_TRY(launch_physical(*c, f, c->phys_c, c->phys_r[0]));
   const size_t n = (size_t)c->d.Ny * c->d.Mz * 2 * c->d.Mx;
   CHF_TRY(hipMemcpyAsync(out, c->phys_r[0], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CHF_TRY(hipStreamSynchronize(c->stream));
@@ -858,7 +882,7 @@ int chf_statistics(void* ctx, double* out) {
   if (!out) return fail(c, "chf_statistics: null output");
   const int rc = ensure_physical(c);
   if (rc) return rc;
-  for (int q = 0; q < 3; ++q) CHF_TRY(launch_physical(*c, field_ptr(c, q), c->phys_r[q]));
+  for (int q = 0; q < 3; ++q) CHF_TRY(launch_physical(*c, field_ptr(c, q), c->phys_c, c->phys_r[q]));
   CHF_TRY(launch_plane_stats(*c, c->phys_r[0], c->phys_r[1], c->phys_r[2], c->stats));
   CHF_TRY(hipMemcpyAsync(out, c->stats, 12 * (size_t)c->d.Ny * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream));
@@ -1084,10 +1108,6 @@ int chf_ensemble_step(void* ens, int n) {
   if (n < 0) return ens_fail(e, "chf_ensemble_step: negative step count");
   for (Ctx* c : e->m) {
     if (c->timing) return ens_fail(e, "chf_ensemble_step: turn member timing off first");
-    if (c->prep_stale) {  // brackets after a state change: computed once, outside the graph
-      ENS_TRY(launch_rhs_prep(*c));
-      c->prep_stale = false;
-    }
     // member work queued outside the graph (set_state, init) must finish first
     ENS_TRY(hipStreamSynchronize(c->stream));
   }
@@ -1106,6 +1126,11 @@ int chf_ensemble_step(void* ens, int n) {
     for (Ctx* c : e->m) c->hcur = 1 - c->hcur;  // what nonlinear() does per step
   }
   ENS_TRY(hipStreamSynchronize(e->origin));
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    const int rc = check_finite(e->m[i], "chf_ensemble_step");
+    if (rc == CHF_NONFINITE) return ens_fail(e, "member " + std::to_string(i) + ": " + e->m[i]->err, rc);
+    if (rc) return ens_fail(e, e->m[i]->err, rc);
+  }
   return 0;
 }
 

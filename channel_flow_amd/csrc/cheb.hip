@@ -236,77 +236,11 @@ __device__ __forceinline__ double col_k2(long col, int Mx, const double* al, con
 // coefficient vectors, so
 //   B_v v = T0 [c D^4 v + (1 - 2 c k^2) D^2 v - k^2 (1 - c k^2) v],
 //   B_w w = T0 [c D^2 w + (1 - c k^2) w],           c = dt / (2 re).
-// k_rhs_prep forms the bracketed coefficient vectors with one backward
-// derivative sweep per column; the right-hand sides are then cos-table
-// syntheses at the interior points plus the AB2 forcing
+// The kinematics pass (k_kinematics) forms the bracketed coefficient vectors
+// g_v, g_w in its backward derivative sweep, whenever v and omega_y change
+// (after each solve, chf_set_state and chf_init_fields); the right-hand sides
+// are then cos-table syntheses at the interior points plus the AB2 forcing
 // 3 dt/2 h^n - dt/2 h^{n-1} (PF:252-257, 335-340), folded by parity.
-struct PrepArgs {
-  const double2 *v, *oy;
-  double2 *gv, *go;
-  const double *al, *be;
-  int Mx, N;
-  long ncol;
-  double c;
-};
-
-// The derivative chains split by degree parity: chain p reads v and omega_y at
-// degrees of parity p only (D^1, D^3 land on parity 1-p, D^2, D^4 back on p),
-// so one thread runs one chain of one column, loads issued 8 degrees ahead.
-__global__ __launch_bounds__(256) void k_rhs_prep(PrepArgs a) {
-  const int p = blockIdx.y;
-  const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cm >= a.ncol) return;
-  const int kz = (int)((unsigned)cm / (unsigned)a.Mx);
-  const int kx = (int)(cm - (long)kz * a.Mx);
-  const double s = a.al[kx] * a.al[kx] + a.be[kz] * a.be[kz];
-  const double c = a.c, c2 = 1.0 - 2.0 * c * s, c0 = s * (1.0 - c * s), cw = 1.0 - c * s;
-  const int N = a.N;
-  // unhalved derivative levels two degrees up (same parity) and the inputs one degree up
-  double2 d1 = {}, d2 = {}, d3 = {}, d4 = {}, e1 = {}, e2 = {};
-  double2 vn = make_double2(0, 0), wn = make_double2(0, 0);  // v_{k+1}, w_{k+1} (parity p)
-  auto step = [](double2 up2, double2 lowp1, int k) {          // b_k = b_{k+2} + 2(k+1) a_{k+1}
-    const double f = 2.0 * (k + 1);
-    return make_double2(up2.x + f * lowp1.x, up2.y + f * lowp1.y);
-  };
-  for (int k0 = N; k0 >= 0; k0 -= 8) {
-    double2 vin[8], win[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int k = k0 - t;
-      if (k >= 0 && (k & 1) == p) {
-        const long off = (long)k * a.ncol + cm;
-        vin[t] = a.v[off];
-        win[t] = a.oy[off];
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int k = k0 - t;
-      if (k < 0) break;
-      if ((k & 1) != p) {
-        // odd levels at degree k from the even levels (and inputs) at k + 1
-        d1 = step(d1, vn, k);
-        d3 = step(d3, d2, k);
-        e1 = step(e1, wn, k);
-      } else {
-        const double2 vk = vin[t], wk = win[t];
-        double2 q2 = step(d2, d1, k), q4 = step(d4, d3, k), r2 = step(e2, e1, k);
-        if (k == N) q2 = q4 = r2 = make_double2(0, 0);
-        d2 = q2;
-        d4 = q4;
-        e2 = r2;
-        if (k == 0) {  // c_0 = 2
-          q2.x *= 0.5; q2.y *= 0.5; q4.x *= 0.5; q4.y *= 0.5; r2.x *= 0.5; r2.y *= 0.5;
-        }
-        const long off = (long)k * a.ncol + cm;
-        a.gv[off] = make_double2(c * q4.x + c2 * q2.x - c0 * vk.x, c * q4.y + c2 * q2.y - c0 * vk.y);
-        a.go[off] = make_double2(c * r2.x + cw * wk.x, c * r2.y + cw * wk.y);
-        vn = vk;
-        wn = wk;
-      }
-    }
-  }
-}
 
 // rows r = 0.. at points p = r + POFF (POFF = 2 for v, 1 for omega_y)
 template <int POFF>
@@ -351,23 +285,6 @@ struct RhsSynP {
     if (r < no - POFF) outO[r * ncold + col] = acc[1] + 0.5 * (fp - fq);
   }
 };
-
-hipError_t launch_rhs_prep(Ctx& c) {
-  PrepArgs a{};
-  a.v = c.v;
-  a.oy = c.oy;
-  a.gv = c.gv;
-  a.go = c.go;
-  a.al = c.alpha;
-  a.be = c.beta;
-  a.Mx = c.d.Mx;
-  a.N = c.d.N;
-  a.ncol = c.d.ncol;
-  a.c = c.prm.dt * 0.5 / c.prm.re;
-  hipLaunchKernelGGL(k_rhs_prep, dim3((unsigned)((a.ncol + kColumnThreads - 1) / kColumnThreads), 2),
-                     dim3(kColumnThreads), 0, c.stream, a);
-  return hipGetLastError();
-}
 
 template <int POFF>
 static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const double2* hp, double2* oE,

@@ -71,9 +71,11 @@ inline MfmaShape mfma_shape(int M, int K) {
   return s;
 }
 
-// Kernel classes for per-launch timing.  K_MEAN_POINTS has no launch of its
-// own: the mean-mode point values come from the last block of the h-assembly
-// launch and are timed with it (the id is kept so the class ids stay stable).
+// Kernel classes for per-launch timing.  K_MEAN_POINTS and K_RHS_PREP have no
+// launch of their own: the mean-mode point values come from the last block of
+// the h-assembly launch and are timed with it, and the right-hand-side
+// brackets are formed by the kinematics pass (the ids are kept so that the
+// class ids stay stable).
 enum KernelId {
   K_SYNTH_PADDED = 0, K_ZFFT_INV, K_XPASS, K_ZFFT_FWD, K_ANALYSIS, K_HASSEMBLE, K_SYNTH_POINTS,
   K_MEAN_POINTS, K_RHS_V, K_RHS_O, K_SOLVE, K_RHS_PREP, K_KINEMATICS, K_WSOLVE, K_COUNT
@@ -115,9 +117,12 @@ struct Ctx {
   // solve work
   double2 *rvE = nullptr, *rvO = nullptr, *roE = nullptr, *roO = nullptr;
   double2 *gv = nullptr, *go = nullptr;  // explicit-operator coefficient vectors
-  // gv/go are written by the kinematics pass for the next solve; after the
-  // state is replaced (chf_set_state) a standalone pass recomputes them
-  bool prep_stale = true;
+  // gv/go are written by the kinematics pass for the next solve (it runs after
+  // every solve and whenever the state is replaced), so a resumed run forms
+  // them with the same kernel as an uninterrupted one
+  // device flag: the last kinematics pass saw a NaN or an infinity in v,
+  // omega_y, U or W (cleared when a new state is loaded)
+  int* nonfinite = nullptr;
   // operators
   double* host_mats = nullptr;
   double *tsyn = nullptr, *tana = nullptr, *tpts = nullptr, *t0 = nullptr, *t2 = nullptr,
@@ -198,9 +203,9 @@ hipError_t launch_hassemble(Ctx& c);
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);
 hipError_t launch_stream_c2r_unpadded(Ctx& c, const double2* in, double* out);
-hipError_t launch_physical(Ctx& c, const double2* field, double* out);
+// unpadded physical field: work = complex [Ny][Mz][Mx], out = real [Ny][nny][nnx]
+hipError_t launch_physical(Ctx& c, const double2* field, double2* work, double* out);
 hipError_t launch_plane_stats(Ctx& c, const double* u, const double* v, const double* w, double* out);
-hipError_t launch_rhs_prep(Ctx& c);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
 hipError_t launch_solve(Ctx& c);       // v, omega_y and the mean mode
 hipError_t launch_wsolve(Ctx& c);      // diagonalised omega_y solve (all modes but the mean)

@@ -429,6 +429,7 @@ struct KinArgs {
   double2 *u, *w, *ox, *oz;
   double2 *gv, *go;  // brackets for the next solve
   const double *al, *be;
+  int* nonfinite;    // set to 1 when v, omega_y, U or W hold a NaN or an infinity
   int Mx, Mz, N, Ny;
   long ncol;
   double c;          // dt / (2 re)
@@ -443,10 +444,13 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
     extern __shared__ double sm[];
     double* x = sm;              // U.re U.im W.re W.im   [4][Ny]
     double* d = sm + 4 * a.Ny;   // DU.re DU.im DW.re DW.im
+    bool finite = true;
     for (int k = threadIdx.x; k < a.Ny; k += blockDim.x) {
       const double2 U = a.au0[k], W = a.w0[k];
       x[0 * a.Ny + k] = U.x; x[1 * a.Ny + k] = U.y; x[2 * a.Ny + k] = W.x; x[3 * a.Ny + k] = W.y;
+      finite = finite && isfinite(U.x) && isfinite(U.y) && isfinite(W.x) && isfinite(W.y);
     }
+    if (!finite) a.nonfinite[0] = 1;
     __syncthreads();
     if (threadIdx.x < 4) {
       const double* xa = x + threadIdx.x * a.Ny;
@@ -485,12 +489,16 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
   double2 d2s = make_double2(0, 0), d3s = make_double2(0, 0), d4s = make_double2(0, 0);
   double2 e1s = make_double2(0, 0), e2s = make_double2(0, 0), wn = make_double2(0, 0);
   const double cc = a.c, c2 = 1.0 - 2.0 * cc * k2, c0 = k2 * (1.0 - cc * k2), cw = 1.0 - cc * k2;
+  bool finite = true;
   for (int k0 = N; k0 >= 0; k0 -= 8) {
     double2 in[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const int k = k0 - s;
-      if (k >= 0) in[s] = (((k & 1) == q) ? a.v : a.oy)[(long)k * a.ncol + cm];
+      if (k >= 0) {
+        in[s] = (((k & 1) == q) ? a.v : a.oy)[(long)k * a.ncol + cm];
+        finite = finite && isfinite(in[s].x) && isfinite(in[s].y);
+      }
     }
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -547,6 +555,7 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
       }
     }
   }
+  if (!finite) a.nonfinite[0] = 1;
 }
 
 // v from the solve's per-group records (record g = |kz| Mx + kx holds
@@ -843,6 +852,7 @@ hipError_t launch_kinematics(Ctx& c) {
   a.go = c.go;
   a.al = c.alpha;
   a.be = c.beta;
+  a.nonfinite = c.nonfinite;
   a.Mx = d.Mx;
   a.Mz = d.Mz;
   a.N = d.N;

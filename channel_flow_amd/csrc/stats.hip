@@ -73,15 +73,16 @@ __global__ __launch_bounds__(256) void k_plane_stats(const double* __restrict__ 
   }
 }
 
-hipError_t launch_physical(Ctx& c, const double2* field, double* out) {
-  // Chebyshev synthesis at the N+1 points into the complex work plane
+hipError_t launch_physical(Ctx& c, const double2* field, double2* work, double* out) {
+  // Chebyshev synthesis at the N+1 points into the complex work plane (every
+  // point of every column is written, so the work needs no clearing)
   const double2* in1[1] = {field};
-  double2* o1[1] = {c.phys_c};
+  double2* o1[1] = {work};
   hipError_t e = launch_synth_points(c, in1, o1, 1);
   if (e != hipSuccess) return e;
-  e = launch_span_inverse_unpadded(c, c.phys_c);
+  e = launch_span_inverse_unpadded(c, work);
   if (e != hipSuccess) return e;
-  return launch_stream_c2r_unpadded(c, c.phys_c, out);
+  return launch_stream_c2r_unpadded(c, work, out);
 }
 
 hipError_t launch_plane_stats(Ctx& c, const double* u, const double* v, const double* w, double* out) {

@@ -33,7 +33,7 @@ class Ensemble:
         rc = self._lib.chf_ensemble_create(arr, len(members), ctypes.byref(self._ens))
         if rc != 0:
             msg = self._lib.chf_ensemble_last_error(None).decode()
-            raise rt.ChfError(f"chf_ensemble_create failed ({rc}): {msg}")
+            raise rt.error(f"chf_ensemble_create failed ({rc}): {msg}", rc)
         self._members = []
         for i in range(len(members)):
             ctx = ctypes.c_void_p()
@@ -43,7 +43,7 @@ class Ensemble:
     def _check(self, rc, what):
         if rc != 0:
             msg = self._lib.chf_ensemble_last_error(self._ens).decode()
-            raise rt.ChfError(f"{what} failed ({rc}): {msg}")
+            raise rt.error(f"{what} failed ({rc}): {msg}", rc)
 
     def __len__(self):
         return len(self._members)

@@ -8,7 +8,11 @@
 ! (output, PF:2460-2504) and then the per-mode and mean-mode solves
 ! (PF:242-474).  The restart files use the reference's own record layout and
 ! name, data00000002_instant.000.dat (PF:2466-2480), so either program can
-! continue the other's run.
+! continue the other's run.  Every `outputstep` steps, `statistics` writes the
+! reference's output files under output_dir (PF:2666-3198): the physical
+! velocity dump 3Ddata/velocity<istep>.000.dat and energy.plt, and at steps
+! that are also multiples of `interupt` the meanu/, stress/, rms/ and uplus/
+! profiles and diagnostics.plt.  The directories are created when missing.
 program poiseuilleflow_gpu
   use, intrinsic :: iso_c_binding
   use paraandcons
@@ -28,6 +32,7 @@ program poiseuilleflow_gpu
     call read_parameters(trim(arg))
   end if
   call allocate_state()
+  call make_output_dirs()
 
   prm%nnx = nnx; prm%nny = nny; prm%nnz = nnz
   prm%aphi = aphi; prm%beta = beta; prm%dt = dt; prm%re = re; prm%re_p = re_p
@@ -55,7 +60,7 @@ program poiseuilleflow_gpu
   call system_clock(c0, crate)
   do istep = istepbegin, nstep                  ! PF:220
     if (istep /= istepbegin) then
-      if (mod(istep, outputstep) == 0) call mean_statistics(istep)   ! statistics, PF:223
+      if (mod(istep, outputstep) == 0) call statistics(istep)   ! PF:223
       call chf_check(chf_nonlinear(ctx), ctx, 'chf_nonlinear')   ! PF:224
       if (mod(istep, interupt) == 0) then        ! output, PF:225 / 2465
         call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
@@ -117,15 +122,65 @@ contains
     call chf_check(chf_get_state(ctx, st), ctx, 'chf_get_state')
   end subroutine laminar_start
 
-  ! The printed diagnostics of statistics (PF:2816-2868, 3080-3173) from the
-  ! plane statistics reduced on the GPU: CFL numbers, wall shear velocities
-  ! from the total shear stress -<u'v'> + dU/dy / re at the walls, Re_tau,
-  ! Re_c and Re_b.
-  subroutine mean_statistics(istep)
+  ! output_dir and its 3Ddata, meanu, stress, rms and uplus folders, which the
+  ! reference expects to exist (mkdir(2) through the C library; an existing
+  ! folder is left as it is)
+  subroutine make_output_dirs()
+    interface
+      integer(c_int) function c_mkdir(path, mode) bind(C, name="mkdir")
+        import :: c_int, c_char
+        character(kind=c_char), intent(in) :: path(*)
+        integer(c_int), value :: mode
+      end function c_mkdir
+    end interface
+    character(len=16), parameter :: sub(5) = [character(len=16) :: '3Ddata', 'meanu', 'stress', 'rms', 'uplus']
+    integer :: i, rc
+    rc = c_mkdir(trim(output_dir) // c_null_char, int(o'755', c_int))
+    do i = 1, 5
+      rc = c_mkdir(trim(output_dir) // '/' // trim(sub(i)) // c_null_char, int(o'755', c_int))
+    end do
+  end subroutine make_output_dirs
+
+  ! statistics (PF:2666-3198) from the GPU's physical fields (chf_physical,
+  ! transform(1, .) of PF:2723-2725) and plane reductions (chf_statistics,
+  ! PF:2816-3020).  Every output step: the physical velocity (u + u0, w, v)
+  ! as one unformatted record of real*8 (0:nx,0:ny,0:nz,3) in
+  ! 3Ddata/velocity<istep>.000.dat (PF:2749-2764), and istep with the summed
+  ! kinetic energy appended to energy.plt (PF:2766-2787).  At steps that are
+  ! also multiples of interupt: the CFL numbers (PF:2816-2934), the profiles
+  ! meanu/average_velocity, stress/{Reynolds_shear,viscous,total_shear}_stress,
+  ! rms/{u,v,w}rms and uplus/uplus (PF:3073-3159), the printed wall shear
+  ! velocities and Reynolds numbers, and diagnostics.plt (PF:3161-3189).  The
+  ! reference's base flow u0, du0 is zero (SCHEB, PF:3319-3321), so u0 adds
+  ! nothing.
+  subroutine statistics(istep)
     integer, intent(in) :: istep
-    real*8 :: ps(0:nz, 12), dudy(0:nz), tau(0:nz), zz(0:nz), dy(0:nz)
-    real*8 :: pi, dx, ddz, ddy, cfl_x, cfl_y, cfl_z, ubulk, ut_b, ut_t
-    integer :: k
+    real*8, allocatable :: ftmp(:,:,:,:)
+    real*8 :: ps(0:nz, 12), dudy(0:nz), tau(0:nz), zz(0:nz), dy(0:nz), yplus(0:nz), sumn(0:nz)
+    real*8 :: pi, dx, ddz, ddy, cfl_x, cfl_y, cfl_z, ubulk, ut_b, ut_t, sume
+    character(len=600) :: fname
+    integer :: k, u
+
+    allocate(ftmp(0:nx, 0:ny, 0:nz, 3))
+    call chf_check(chf_physical(ctx, 0, ftmp(:, :, :, 1)), ctx, 'chf_physical(u)')
+    call chf_check(chf_physical(ctx, 2, ftmp(:, :, :, 2)), ctx, 'chf_physical(w)')
+    call chf_check(chf_physical(ctx, 1, ftmp(:, :, :, 3)), ctx, 'chf_physical(v)')
+    write(fname, '(a,a,i8.8,a,i3.3,a)') trim(output_dir), '/3Ddata/velocity', istep, '.', 0, '.dat'
+    open(newunit=u, file=trim(fname), status='replace', form='unformatted')
+    write(u) ftmp
+    close(u)
+    sume = 0.5d0 * sum(ftmp * ftmp)
+    deallocate(ftmp)
+    if (interupt >= 10) then
+      if (mod(istep, interupt / 10) == 0) then
+        write(*, '(a,i8,3x,a,f15.3,3x,a,es16.8)') 'istep=', istep, 'time=', dble(istep) * dt, 'engery=', sume
+      end if
+    end if
+    open(newunit=u, file=trim(output_dir) // '/energy.plt', form='formatted', position='append')
+    write(u, *) istep, sume
+    close(u)
+
+    if (mod(istep, interupt) /= 0) return
     call chf_check(chf_statistics(ctx, ps), ctx, 'chf_statistics')
     pi = 4.0d0 * atan(1.0d0)
     dx = aphi * 2.0d0 * pi / dble(nnx)
@@ -145,23 +200,65 @@ contains
     do k = 1, nz - 1
       cfl_y = max(cfl_y, ps(k, 11) * dt / dy(k))
     end do
+    write(*, '(a,es14.6,a,es14.6,a,es14.6)') 'CFL_X= ', cfl_x, ' CFL_Y= ', cfl_y, ' CFL_Z= ', cfl_z
+
+    sumn = ps(:, 1)                               ! ua + u0
     call derivative_at_points(ps(:, 1), dudy)
-    tau = -ps(:, 7) + dudy / re
+    tau = -ps(:, 7) + dudy / re                   ! total shear stress (PF:3086-3088)
+
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/meanu/average_velocity', istep, '.plt'
+    call write_profile(fname, zz, sumn, nz + 1)
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/stress/Reynolds_shear_stress', istep, '.plt'
+    call write_profile(fname, zz, -ps(:, 7) / tau(nz), nz + 1)
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/stress/viscous_stress', istep, '.plt'
+    call write_profile(fname, zz, (dudy / re) / tau(nz), nz + 1)
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/stress/total_shear_stress', istep, '.plt'
+    call write_profile(fname, zz, tau / tau(nz), nz + 1)
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/rms/urms', istep, '.plt'
+    call write_profile(fname, zz, sqrt(ps(:, 4)) / sqrt(tau(nz)), nz + 1)
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/rms/vrms', istep, '.plt'
+    call write_profile(fname, zz, sqrt(ps(:, 5)) / sqrt(tau(nz)), nz + 1)
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/rms/wrms', istep, '.plt'
+    call write_profile(fname, zz, sqrt(ps(:, 6)) / sqrt(tau(nz)), nz + 1)
+    do k = 0, nz / 2
+      yplus(k) = re * ((1.0d0 - zz(k)) + (1.0d0 + zz(nz - k))) / 2.0d0 * sqrt(tau(nz))
+    end do
+    write(fname, '(a,a,i8.8,a)') trim(output_dir), '/uplus/uplus', istep, '.plt'
+    call write_profile(fname, yplus, [((sumn(k) + sumn(nz - k)) / 2.0d0 / sqrt(tau(nz)), k = 0, nz / 2)], &
+                       nz / 2 + 1)
+
     ut_b = sqrt(abs(tau(nz)))
     ut_t = sqrt(abs(tau(0)))
     ubulk = 0.0d0
     do k = nz / 2, nz - 1
-      ubulk = ubulk + ps(k, 1) * abs(zz(k + 1) - zz(k))
+      ubulk = ubulk + sumn(k) * abs(zz(k + 1) - zz(k))
     end do
     write(*, '(a,i8)') 'istep=', istep
-    write(*, '(a,es14.6,a,es14.6,a,es14.6)') 'CFL_X= ', cfl_x, ' CFL_Y= ', cfl_y, ' CFL_Z= ', cfl_z
     write(*, '(a,2es14.6)') 'wall shear velocity: ', ut_b, ut_t
     write(*, '(a,es14.6,a,2es14.6)') 'Mean pressure gradient[d<p>/dx]: ', -re_p, &
         ' & The shear-stress gradient: ', -tau(nz), tau(0)
     write(*, '(a,2es14.6)') 'Reynolds number R_t: ', ut_b * re, ut_t * re
-    write(*, '(a,es14.6)') 'Reynolds number R_c: ', ps(nz / 2, 1) * re
+    write(*, '(a,es14.6)') 'Reynolds number R_c: ', sumn(nz / 2) * re
     write(*, '(a,es14.6)') 'Reynolds number R_b: ', 2.0d0 * ubulk * re
-  end subroutine mean_statistics
+    open(newunit=u, file=trim(output_dir) // '/diagnostics.plt', form='formatted', position='append')
+    write(u, *) istep, ut_b, ut_t, -re_p, -tau(nz), tau(0), ut_b * re, ut_t * re, sumn(nz / 2) * re, &
+        2.0d0 * ubulk * re
+    close(u)
+  end subroutine statistics
+
+  ! one profile file: a line "x(k) f(k)" per point, list-directed as the
+  ! reference writes them (PF:3073-3159)
+  subroutine write_profile(fname, x, f, n)
+    character(len=*), intent(in) :: fname
+    integer, intent(in) :: n
+    real*8, intent(in) :: x(n), f(n)
+    integer :: k, u
+    open(newunit=u, file=trim(fname), status='replace')
+    do k = 1, n
+      write(u, *) x(k), f(k)
+    end do
+    close(u)
+  end subroutine write_profile
 
   ! d/dy of the degree-nz interpolant of f(y_k) at y_k = cos(pi k/nz): the
   ! chebyshev(-1) -> dz -> chebyshev(1) sequence of PF:3083-3085.

@@ -106,5 +106,20 @@ def dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+NONFINITE = -2        # CHF_NONFINITE: the state holds a NaN or an infinity
+
+
 class ChfError(RuntimeError):
-    pass
+    """A failed libchf call; `code` is its status (see include/chf.h)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+class ChfNonFinite(ChfError):
+    """The state became (or was loaded) non-finite: a blow-up."""
+
+
+def error(msg, code):
+    return (ChfNonFinite if code == NONFINITE else ChfError)(msg, code)

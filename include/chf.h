@@ -13,8 +13,9 @@
  *                             dim 1 == complex128[Ny][Mz][Mx], C order,
  *                             Mx = nnx/2, Mz = nny, Ny = nnz+1;
  *       mean-mode array       real*8 (2,0:nz) == complex128[Ny].
- *   - Return value: 0 ok; < 0 invalid argument (message in chf_last_error);
- *     > 0 a HIP runtime error code.
+ *   - Return value: 0 ok; CHF_NONFINITE when the state holds a NaN or an
+ *     infinity (below); other values < 0: invalid argument (message in
+ *     chf_last_error); > 0 a HIP runtime error code.
  *   - One context = one channel on the current HIP device and one stream.
  *     Calls on one context must come from one host thread.
  */
@@ -24,6 +25,13 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Status of chf_set_state, chf_init_fields, chf_solve, chf_step and
+ * chf_ensemble_step when v, omega_y, U or W hold a NaN or an infinity after
+ * the call (a blow-up, or a non-finite state loaded).  The reference has no
+ * such check: it ignores every status and prints CFL numbers only
+ * (PF:201-204, 2933).  The state is left as it is. */
+#define CHF_NONFINITE (-2)
 
 /* Compile-time parameters of PC:4-26 that the timestep core reads (P = 1). */
 typedef struct chf_params {
@@ -49,13 +57,20 @@ typedef struct chf_state {
 } chf_state;
 
 /* Builds the per-(kx,|kz|) operator inverses on the GPU (replaces SCHEB PF:156
- * and the get_coef_v/get_coef_omegay + zgetrf/zgetri set-up loop, PF:181-204). */
+ * and the get_coef_v/get_coef_omegay + zgetrf/zgetri set-up loop, PF:181-204).
+ * The omega_y operator is diagonalised when the decomposition reproduces
+ * direct solves to 1e-12, else its half-system inverses are stored;
+ * CHF_OMEGA_INVERSES=stored in the environment selects the stored inverses. */
 int chf_create(const chf_params* prm, void** ctx);
 int chf_destroy(void* ctx);
 const char* chf_last_error(void* ctx);
 
 /* Host -> device copy of a pre-solve state (replaces initial(isign=1),
- * PF:1861-1924). */
+ * PF:1861-1924).  Derives u, w, omega_x, omega_z from the loaded v, omega_y,
+ * U, W (getvelocityandvorticity, PF:2347-2407), so chf_get_fields,
+ * chf_physical and chf_statistics describe the loaded state, and a run
+ * resumed from a chf_get_state record continues bit for bit as the
+ * uninterrupted run does. */
 int chf_set_state(void* ctx, const chf_state* st);
 /* Device -> host copy of the pre-solve state (replaces output, PF:2460-2504). */
 int chf_get_state(void* ctx, const chf_state* st);
@@ -70,9 +85,12 @@ int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1,
 /* getnonlinear (PF:1380-1612): six dealiased products -> Lamb vector ->
  * h_v, h_g, AB2 history rotation, point values. */
 int chf_nonlinear(void* ctx);
-/* Per-mode v / omega_y solves, kinematics and the mean mode (PF:242-474). */
+/* Per-mode v / omega_y solves, kinematics and the mean mode (PF:242-474).
+ * Returns after the solve has completed (CHF_NONFINITE on a blow-up). */
 int chf_solve(void* ctx);
-/* n x [chf_solve ; chf_nonlinear] from a pre-solve state (PF:220-532). */
+/* n x [chf_solve ; chf_nonlinear] from a pre-solve state (PF:220-532).
+ * Returns after the steps have completed (CHF_NONFINITE when the state after
+ * the last solve is not finite). */
 int chf_step(void* ctx, int n);
 /* Waits for all work queued on the context's stream. */
 int chf_sync(void* ctx);

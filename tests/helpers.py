@@ -65,12 +65,16 @@ SMALL = dict(nnx=16, nny=16, nnz=16, aphi=1.0, beta=0.5, re=500.0, re_p=2.0 / 50
              u_c=1.0, rms=5.0e-2, kmax=4.0, slope=0.0, seed=7)
 
 
-def solved_state_invariants(p, st, fl):
+def solved_state_invariants(p, st, fl, before=None):
     """Relative residuals of the properties every solved state has: continuity
     iαu + Dv + iβw = 0 and ω_y = iβu − iαw for k² ≠ 0 (PF:2357-2403), the
-    Hermitian kx = 0 plane, the unsolved spanwise Nyquist column and the
-    frozen v(0,0), ω_y(0,0) (PF:247, 2365).  `st` holds v, ω_y; `fl` holds
-    the post-solve u, w."""
+    Hermitian kx = 0 plane, the mean-mode vorticity ω_x(0,0) = DW,
+    ω_z(0,0) = −DU with u(0,0) = U, w(0,0) = W (PF:423-472), the unsolved
+    spanwise Nyquist column and the frozen v(0,0), ω_y(0,0) (PF:247, 2365).
+    `st` holds v, ω_y, U, W; `fl` the post-solve u, w, ω_x, ω_z.  Without
+    `before` the Nyquist column and the (0,0) modes are measured as they are
+    (zero for a seeded start); with `before` (the state the run started from)
+    as their change, which a frozen entry never has."""
     alpha, beta = O.wavenumbers(p)
     a3 = alpha[None, None, :]
     b3 = beta[None, :, None]
@@ -87,11 +91,20 @@ def solved_state_invariants(p, st, fl):
     jr = (mz - np.arange(1, mz // 2)) % mz
     herm = max(float(np.max(np.abs(f[:, jr, 0] - np.conj(f[:, 1:mz // 2, 0]))))
                / float(np.max(np.abs(f))) for f in (v, oy))
+    dU, dW = O.dz(st["au0"]), O.dz(st["w0"])
+    mscale = max(float(np.max(np.abs(dU))), float(np.max(np.abs(dW))), 1e-300)
+    mean_curl = max(float(np.max(np.abs(fl["omegaxnp1"][:, 0, 0] - dW))),
+                    float(np.max(np.abs(fl["omegaznp1"][:, 0, 0] + dU)))) / mscale
+    mean_vel = max(float(np.max(np.abs(u[:, 0, 0] - st["au0"]))), float(np.max(np.abs(w[:, 0, 0] - st["w0"]))))
+    ref = before if before is not None else {k: np.zeros_like(st[k]) for k in ("vnp1", "omegaynp1")}
     return {
         "divergence": float(np.max(np.abs(div))) / max(vs, us),
         "omega_y_curl": float(np.max(np.abs(curl))) / float(np.max(np.abs(oy))),
         "hermitian_kx0": herm,
-        "nyquist": max(float(np.max(np.abs(v[:, mz // 2, :]))),
-                       float(np.max(np.abs(oy[:, mz // 2, :])))),
-        "frozen_00": max(float(np.max(np.abs(v[:, 0, 0]))), float(np.max(np.abs(oy[:, 0, 0])))),
+        "mean_curl": mean_curl,
+        "mean_velocity": mean_vel,
+        "nyquist": max(float(np.max(np.abs(v[:, mz // 2, :] - ref["vnp1"][:, mz // 2, :]))),
+                       float(np.max(np.abs(oy[:, mz // 2, :] - ref["omegaynp1"][:, mz // 2, :])))),
+        "frozen_00": max(float(np.max(np.abs(v[:, 0, 0] - ref["vnp1"][:, 0, 0]))),
+                         float(np.max(np.abs(oy[:, 0, 0] - ref["omegaynp1"][:, 0, 0])))),
     }

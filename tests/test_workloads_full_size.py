@@ -12,8 +12,9 @@ own shapes: 128x129x128 (192x257x192 padded grid), 256x193x256 (384x385x384,
   plus the mean mode, FP64, <= 1e-11 (same two sizes);
 * size-independent properties after 5 steps in both precisions, at all
   three sizes: the continuity iαu + Dv + iβw = 0 and ω_y = iβu − iαw of
-  every solved mode, the Hermitian kx = 0 plane, the unsolved spanwise
-  Nyquist column and the frozen v(0,0), ω_y(0,0) (PF:247, 2365); the mixed
+  every solved mode, the Hermitian kx = 0 plane, the mean-mode vorticity
+  (curl of U, W), the unsolved spanwise Nyquist column and the frozen
+  v(0,0), ω_y(0,0) unchanged from the start (PF:247, 2365); the mixed
   form against the FP64 form at the bounds of the small-box mixed test.
 
 The oracle's initial state at 256x193x256 takes tens of seconds of numpy on
@@ -113,12 +114,14 @@ def test_invariants_and_mixed_form(require_gpu, name):
             ch.step(5)
             st = ch.get_state()
             fl = ch.fields()
-        inv = solved_state_invariants(p, st, fl)
+        inv = solved_state_invariants(p, st, fl, before={"vnp1": arrays[0], "omegaynp1": arrays[1]})
         report(f"{name} {precision} 5 steps", inv)
         assert all(np.all(np.isfinite(st[k])) for k in ("vnp1", "omegaynp1", "au0"))
         assert inv["divergence"] < 1e-12, inv
         assert inv["omega_y_curl"] < 1e-12, inv
         assert inv["hermitian_kx0"] < herm_tol[precision], inv
+        assert inv["mean_curl"] < 1e-12 and inv["mean_velocity"] == 0.0, inv
+        # the Nyquist column and v(0,0), omega_y(0,0) are never solved: unchanged from the start
         assert inv["nyquist"] == 0.0, inv
         assert inv["frozen_00"] == 0.0, inv
         states[precision] = {k: st[k] for k in ("vnp1", "omegaynp1", "au0", "w0")}
