@@ -161,6 +161,14 @@ class Channel:
         self._check(self._lib.chf_physical(self._ctx, i, self._dp(out)), "chf_physical")
         return out
 
+    def physical_noalloc(self, field, out_device_ptr: int):
+        """chf_physical into caller-owned device memory (Ny * nny * nnx
+        doubles at the device address `out_device_ptr`), allocating nothing;
+        queued on the context's stream (sync() waits for it)."""
+        i = self.FIELDS.index(field) if isinstance(field, str) else int(field)
+        self._check(self._lib.chf_physical_noalloc(self._ctx, i, ctypes.c_void_p(out_device_ptr)),
+                    "chf_physical_noalloc")
+
     def plane_statistics(self) -> np.ndarray:
         """The reductions of statistics (PF:2816-3020): double[12][Ny], rows
         ua va wa uu vv ww uv uw vw max|u| max|v| max|w|."""

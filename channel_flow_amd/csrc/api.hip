@@ -6,6 +6,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "chf_internal.h"
@@ -22,6 +23,32 @@ const char* const kKernelNames[K_COUNT] = {
 namespace {
 
 thread_local std::string g_create_err;
+
+// Work memory of chf_physical_noalloc: the complex [Ny][Mz][Mx] plane of the
+// unpadded physical transform, in a file-scope device array so that the
+// entry point allocates nothing.  Each GPU holds its own copy for the life of
+// the process; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the device (never through the array's
+// name, which on the host is a stand-in, not a device address).
+constexpr size_t kScratchBytes = size_t(128) << 20;
+__device__ __align__(256) double2 scratch[kScratchBytes / sizeof(double2)];
+std::mutex g_scratch_mu;
+std::map<int, double2*> g_scratch_of_device;
+
+// The device's scratch address, looked up once per device with that device
+// current (chf_create runs on the caller's current device).
+hipError_t resolve_scratch(int device, double2** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_of_device.find(device);
+  if (it == g_scratch_of_device.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    it = g_scratch_of_device.emplace(device, static_cast<double2*>(p)).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
 
 #define CHF_TRY(expr)                                                   \
   do {                                                                  \
@@ -333,6 +360,7 @@ int build(Ctx* c) {
   d.ngroups = d.Mx * (d.Mz / 2);
 
   CHF_TRY(hipGetDevice(&c->device));
+  CHF_TRY(resolve_scratch(c->device, &c->scratch));
   int max_lds = 0;
   CHF_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
   {
@@ -525,6 +553,15 @@ int build(Ctx* c) {
     CHF_TRY(dalloc(c, &c->twzf, d.Lz));
     CHF_TRY(hipMemcpy(c->twxf, txf.data(), d.Lx * sizeof(float2), hipMemcpyHostToDevice));
     CHF_TRY(hipMemcpy(c->twzf, tzf.data(), d.Lz * sizeof(float2), hipMemcpyHostToDevice));
+  }
+
+  // twiddles of the unpadded physical transform (lengths nnx and nny)
+  {
+    auto tx = twiddles(2 * d.Mx), tz = twiddles(d.Mz);
+    CHF_TRY(dalloc(c, &c->twxu, tx.size()));
+    CHF_TRY(dalloc(c, &c->twzu, tz.size()));
+    CHF_TRY(hipMemcpy(c->twxu, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice));
+    CHF_TRY(hipMemcpy(c->twzu, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice));
   }
 
   // zero the state
@@ -830,17 +867,6 @@ int ensure_physical(Ctx* c) {
   const size_t plane = (size_t)d.Ny * d.Mz * 2 * d.Mx;
   CHF_TRY(dalloc(c, &c->phys_c, (size_t)d.Ny * d.ncol));
   for (auto& p : c->phys_r) CHF_TRY(dalloc(c, &p, plane));
-  auto tw = [](int L) {
-    std::vector<double2> t(L);
-    for (int m = 0; m < L; ++m)
-      t[m] = make_double2(cospi_frac(2L * m, L), -std::sin(2.0 * M_PI * (double)m / (double)L));
-    return t;
-  };
-  const auto tx = tw(2 * d.Mx), tz = tw(d.Mz);
-  CHF_TRY(dalloc(c, &c->twxu, tx.size()));
-  CHF_TRY(dalloc(c, &c->twzu, tz.size()));
-  CHF_TRY(hipMemcpy(c->twxu, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice));
-  CHF_TRY(hipMemcpy(c->twzu, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice));
   CHF_TRY(dalloc(c, &c->stats, 12 * (size_t)d.Ny));
   return 0;
 }
@@ -873,6 +899,21 @@ int chf_physical(void* ctx, int field, double* out) {
   const size_t n = (size_t)c->d.Ny * c->d.Mz * 2 * c->d.Mx;
   CHF_TRY(hipMemcpyAsync(out, c->phys_r[0], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   CHF_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int chf_physical_noalloc(void* ctx, int field, double* out) {
+  if (!ctx) return -1;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  const double2* f = field_ptr(c, field);
+  if (!f) return fail(c, "chf_physical_noalloc: field must be 0..5 (u, v, w, omega_x, omega_y, omega_z)");
+  if (!out) return fail(c, "chf_physical_noalloc: null output");
+  const size_t need = (size_t)c->d.Ny * c->d.ncol * sizeof(double2);
+  if (need > kScratchBytes)
+    return fail(c, "chf_physical_noalloc: the transform needs " + std::to_string(need) +
+                       " bytes of work memory (Ny * nny * nnx/2 complex values), more than its " +
+                       std::to_string(kScratchBytes) + "-byte scratch; use chf_physical");
+  CHF_TRY(launch_physical(*c, f, c->scratch, out));
   return 0;
 }
 

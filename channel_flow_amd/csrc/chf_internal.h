@@ -156,7 +156,8 @@ struct Ctx {
   double2* phys_c = nullptr;                          // [Ny][Mz][Mx] work
   double* phys_r[3] = {nullptr, nullptr, nullptr};    // [Ny][Mz][2 Mx] physical u, v, w
   double* stats = nullptr;                            // [12][Ny]
-  double2 *twxu = nullptr, *twzu = nullptr;           // unpadded twiddles (2 Mx, Mz)
+  double2 *twxu = nullptr, *twzu = nullptr;           // unpadded twiddles (2 Mx, Mz), set up at creation
+  double2* scratch = nullptr;  // this device's no-allocation work memory (shared by its contexts)
   long long bytes = 0;
   std::string err;
   // column blocks per wave of each cos-table contraction, picked at creation

@@ -19,7 +19,8 @@ EXPORTS = ("chf_create", "chf_destroy", "chf_last_error", "chf_set_state", "chf_
            "chf_get_fields", "chf_dims", "chf_device_bytes", "chf_set_timing", "chf_kernel_count",
            "chf_kernel_name", "chf_timing", "chf_ensemble_create", "chf_ensemble_destroy",
            "chf_ensemble_last_error", "chf_ensemble_size", "chf_ensemble_member", "chf_ensemble_step",
-           "chf_physical", "chf_statistics", "chf_tile_choice", "chf_set_tile_choice")
+           "chf_physical", "chf_physical_noalloc", "chf_statistics", "chf_tile_choice",
+           "chf_set_tile_choice")
 
 
 class ChfParams(ctypes.Structure):
@@ -75,6 +76,7 @@ def load():
     lib.chf_tile_choice.argtypes = [vp, ctypes.c_int, dp]
     lib.chf_set_tile_choice.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.chf_physical.argtypes = [vp, ctypes.c_int, dp]
+    lib.chf_physical_noalloc.argtypes = [vp, ctypes.c_int, vp]
     lib.chf_statistics.argtypes = [vp, dp]
     lib.chf_ensemble_create.argtypes = [ctypes.POINTER(ChfParams), ctypes.c_int, ctypes.POINTER(vp)]
     lib.chf_ensemble_destroy.argtypes = [vp]

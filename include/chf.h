@@ -108,6 +108,18 @@ int chf_get_fields(void* ctx, double* unp1, double* wnp1, double* omegaxnp1,
  * z_j = 2 pi beta j/nny, y_k = cos(pi k/N). */
 int chf_physical(void* ctx, int field, double* out);
 
+/* chf_physical into caller-owned device memory, allocating nothing (no
+ * counterpart in the reference: it is for callers that cannot allocate, such
+ * as code under HIP graph capture or a tight serving loop).  `out` is a device
+ * pointer to Ny * nny * nnx doubles, laid out as chf_physical's output.  The
+ * transform's complex work plane (Ny * nny * nnx/2 values, 16 bytes each)
+ * lives in a 128 MiB array that the library holds on each GPU for the life of
+ * the process: larger grids are rejected (< 0, use chf_physical).  The work is
+ * queued on the context's stream (chf_sync waits for it).  Every call on one
+ * GPU shares that GPU's array, so calls that overlap on one GPU (from
+ * contexts on different streams or threads) race. */
+int chf_physical_noalloc(void* ctx, int field, double* out);
+
 /* Plane statistics of the physical u, v, w after the last solve (the
  * reductions of statistics, PF:2816-3020): out = double[12][Ny], rows
  * ua va wa uu vv ww uv uw vw max|u| max|v| max|w| per y_k (means, covariances
