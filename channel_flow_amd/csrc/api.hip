@@ -153,7 +153,7 @@ void free_all(Ctx* c) {
   f(c->costabf); f(c->twxf); f(c->twzf);
   for (auto& l : c->Lc) f(l);
   f(c->hvc); f(c->hgc);
-  f(c->rvE); f(c->rvO); f(c->roE); f(c->roO);
+  f(c->rvT); f(c->roE); f(c->roO);
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
   f(c->inv); f(c->inv0); f(c->crep); f(c->bmat); f(c->bg0); f(c->bg1);
   for (int i = 0; i < 2; ++i) { f(c->wL[i]); f(c->wR[i]); f(c->wlam[i]); }
@@ -406,8 +406,7 @@ int build(Ctx* c) {
   for (auto& l : c->Lc) CHF_TRY(dalloc(c, &l, fld));
   CHF_TRY(dalloc(c, &c->hvc, fld));
   CHF_TRY(dalloc(c, &c->hgc, fld));
-  CHF_TRY(dalloc(c, &c->rvE, (size_t)d.ne * d.ncol));
-  CHF_TRY(dalloc(c, &c->rvO, (size_t)d.ne * d.ncol));
+  CHF_TRY(dalloc(c, &c->rvT, (size_t)(d.ne + d.no - 4) * d.ncol));
   CHF_TRY(dalloc(c, &c->roE, (size_t)d.ne * d.ncol));
   CHF_TRY(dalloc(c, &c->roO, (size_t)d.ne * d.ncol));
   CHF_TRY(dalloc(c, &c->gv, fld));

@@ -257,7 +257,9 @@ struct RhsSynP {
   MfmaShape sh;  // M = ne - POFF, K = ne
   const double* g;
   const double *hn, *hp;
-  double *outE, *outO;
+  double *outE, *outO;  // omega_y (POFF = 1): [row][real column] per parity
+  double* outT;         // v (POFF = 2): [complex column][Rv rows: even, then odd][Re, Im]
+  int Rv;
   int N, no;
   long ncold;
   double dt;
@@ -281,14 +283,22 @@ struct RhsSynP {
     const double fp = 3.0 * dt / 2.0 * hn[p * ncold + col] - dt / 2.0 * hp[p * ncold + col];
     const double fq = 3.0 * dt / 2.0 * hn[q * ncold + col] - dt / 2.0 * hp[q * ncold + col];
     const double sym = (p == q) ? fp : 0.5 * (fp + fq);
-    outE[r * ncold + col] = acc[0] + sym;
-    if (r < no - POFF) outO[r * ncold + col] = acc[1] + 0.5 * (fp - fq);
+    if constexpr (POFF == 2) {
+      // a mode's record is [even rows][odd rows]: a block writes the rows of
+      // its columns in 64-byte runs, and the solve reads each mode in one run
+      double* rec = outT + (col >> 1) * (2L * Rv) + (col & 1);
+      rec[2 * r] = acc[0] + sym;
+      if (r < no - POFF) rec[2 * (sh.M + r)] = acc[1] + 0.5 * (fp - fq);
+    } else {
+      outE[r * ncold + col] = acc[0] + sym;
+      if (r < no - POFF) outO[r * ncold + col] = acc[1] + 0.5 * (fp - fq);
+    }
   }
 };
 
 template <int POFF>
 static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const double2* hp, double2* oE,
-                          double2* oO) {
+                          double2* oO, double2* oT) {
   const int cb = c.cb_choice[POFF == 2 ? K_RHS_V : K_RHS_O];
   RhsSynP<POFF> pb{};
   pb.costab = c.costab;
@@ -299,6 +309,8 @@ static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const dou
   pb.hp = reinterpret_cast<const double*>(hp);
   pb.outE = reinterpret_cast<double*>(oE);
   pb.outO = reinterpret_cast<double*>(oO);
+  pb.outT = reinterpret_cast<double*>(oT);
+  pb.Rv = c.d.ne + c.d.no - 4;
   pb.N = c.d.N;
   pb.no = c.d.no;
   pb.ncold = 2 * c.d.ncol;
@@ -308,8 +320,8 @@ static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const dou
 
 hipError_t launch_rhs(Ctx& c, int which) {
   const int nxt = c.hcur, prv = 1 - c.hcur;  // h^n, h^{n-1}
-  if (which == 0) return rhs_syn<2>(c, c.gv, c.hv[nxt], c.hv[prv], c.rvE, c.rvO);
-  return rhs_syn<1>(c, c.go, c.hg[nxt], c.hg[prv], c.roE, c.roO);
+  if (which == 0) return rhs_syn<2>(c, c.gv, c.hv[nxt], c.hv[prv], nullptr, nullptr, c.rvT);
+  return rhs_syn<1>(c, c.go, c.hg[nxt], c.hg[prv], c.roE, c.roO, nullptr);
 }
 
 // ---------------------------------------------------------------- h assembly

@@ -115,7 +115,12 @@ struct Ctx {
   double2* Lc[3] = {};    // [Ny][Mz][Mx]   Lamb vector coefficients
   double2 *hvc = nullptr, *hgc = nullptr;  // h coefficients
   // solve work
-  double2 *rvE = nullptr, *rvO = nullptr, *roE = nullptr, *roO = nullptr;
+  // v right-hand sides per mode: [column][even rows (ne - 2), then odd rows
+  // (no - 2)], so a solve block reads each of its modes' rows contiguously;
+  // omega_y right-hand sides per parity: [row][column] (the omega_y solve
+  // reads 64 neighbouring columns at a time)
+  double2* rvT = nullptr;
+  double2 *roE = nullptr, *roO = nullptr;
   double2 *gv = nullptr, *go = nullptr;  // explicit-operator coefficient vectors
   // gv/go are written by the kinematics pass for the next solve (it runs after
   // every solve and whenever the state is replaced), so a resumed run forms

@@ -177,7 +177,9 @@ hipError_t launch_setup_inverses(Ctx& c) {
 // ------------------------------------------------------------------- solve
 
 struct SolveArgs {
-  const double2 *rvE, *rvO, *roE, *roO;  // parity-folded right-hand sides
+  const double2* rvT;              // v right-hand sides, [column][Rv rows] (even rows, then odd)
+  const double2 *roE, *roO;        // omega_y right-hand sides, [row][column] per parity
+  int Rv;                          // (ne - 2) + (no - 2)
   const double* inv;
   const double* inv_w;
   double2 *vst, *oy;  // v by group (record [l][side], see KinArgs), omega_y
@@ -264,9 +266,16 @@ __global__ __launch_bounds__(512) void k_solve(SolveArgs a) {
         double2 x0 = make_double2(0, 0), x1 = make_double2(0, 0);
         if (i >= nbc) {
           const int r = i - nbc;
-          const double2* src = eq ? (odd ? a.roO : a.roE) : (odd ? a.rvO : a.rvE);
-          x0 = src[(long)r * a.ncol + col0];
-          if (kzab > 0) x1 = src[(long)r * a.ncol + col1];
+          if (eq) {
+            const double2* src = odd ? a.roO : a.roE;
+            x0 = src[(long)r * a.ncol + col0];
+            if (kzab > 0) x1 = src[(long)r * a.ncol + col1];
+          } else {
+            // consecutive rows are consecutive threads: one contiguous run per mode
+            const int k = odd ? (ne - 2) + r : r;
+            x0 = a.rvT[col0 * a.Rv + k];
+            if (kzab > 0) x1 = a.rvT[col1 * a.Rv + k];
+          }
         }
         double* dst = rb + ((gi * neq + eq) * Ny + row) * 4;
         dst[0] = x0.x;
@@ -489,16 +498,15 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
   double2 d2s = make_double2(0, 0), d3s = make_double2(0, 0), d4s = make_double2(0, 0);
   double2 e1s = make_double2(0, 0), e2s = make_double2(0, 0), wn = make_double2(0, 0);
   const double cc = a.c, c2 = 1.0 - 2.0 * cc * k2, c0 = k2 * (1.0 - cc * k2), cw = 1.0 - cc * k2;
-  bool finite = true;
+  // x * 0 is 0 for every finite x and NaN for a NaN or an infinity: one
+  // multiply-add per value, no branch between the batched loads
+  double probe = 0.0;
   for (int k0 = N; k0 >= 0; k0 -= 8) {
     double2 in[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const int k = k0 - s;
-      if (k >= 0) {
-        in[s] = (((k & 1) == q) ? a.v : a.oy)[(long)k * a.ncol + cm];
-        finite = finite && isfinite(in[s].x) && isfinite(in[s].y);
-      }
+      if (k >= 0) in[s] = (((k & 1) == q) ? a.v : a.oy)[(long)k * a.ncol + cm];
     }
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -553,9 +561,10 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
         a.go[off] = make_double2(cc * r2.x + cw * omr, cc * r2.y + cw * omi);
         wn = in[s];
       }
+      probe = fma(in[s].x, 0.0, fma(in[s].y, 0.0, probe));
     }
   }
-  if (!finite) a.nonfinite[0] = 1;
+  if (probe != probe) a.nonfinite[0] = 1;
 }
 
 // v from the solve's per-group records (record g = |kz| Mx + kx holds
@@ -590,8 +599,8 @@ __global__ __launch_bounds__(256) void k_vtranspose(const double2* __restrict__ 
 hipError_t launch_solve(Ctx& c) {
   const Dims& d = c.d;
   SolveArgs a{};
-  a.rvE = c.rvE;
-  a.rvO = c.rvO;
+  a.rvT = c.rvT;
+  a.Rv = d.ne + d.no - 4;
   a.roE = c.roE;
   a.roO = c.roO;
   a.inv = c.inv;
