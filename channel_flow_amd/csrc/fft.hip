@@ -526,15 +526,25 @@ __device__ __forceinline__ int spad(int a) { return a + (a >> 3); }
 //   v[r] = in[j + r L/R] * W_{Ns R}^{r (j mod Ns)},  v = DFT_R(v),
 //   out[(j / Ns) Ns R + (j mod Ns) + r Ns] = v[r].
 // After the last stage the line is in natural order.  `tw` = e^{-2 pi i m / L}.
-template <int L, int R, int Ns, int S, class V>
+// ZIN: inputs r known to be zero for every butterfly (not read); WOUT: outputs
+// r that are written (the others are not needed by anyone).
+template <int L, int R, int Ns, int S, class V, unsigned ZIN = 0u, unsigned WOUT = ~0u>
 __device__ __forceinline__ void st_stage1(const V* in, V* out, const V* tw, int j) {
   V v[R];
   const int k = j % Ns;
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = in[spad(j + r * (L / R))];
+  for (int r = 0; r < R; ++r) {
+    if ((ZIN >> r) & 1u) {
+      v[r].x = 0;
+      v[r].y = 0;
+    } else {
+      v[r] = in[spad(j + r * (L / R))];
+    }
+  }
   if (Ns > 1) {
 #pragma unroll
     for (int r = 1; r < R; ++r) {
+      if ((ZIN >> r) & 1u) continue;
       V w = tw[r * k * (L / (Ns * R))];
       if (S > 0) w.y = -w.y;
       v[r] = cmulv(v[r], w);
@@ -543,14 +553,29 @@ __device__ __forceinline__ void st_stage1(const V* in, V* out, const V* tw, int 
   Dft<R, S, V>::run(v);
   const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-  for (int r = 0; r < R; ++r) out[spad(base + r * Ns)] = v[r];
+  for (int r = 0; r < R; ++r)
+    if ((WOUT >> r) & 1u) out[spad(base + r * Ns)] = v[r];
 }
 
-template <int L, int R, int Ns, int S, class V>
+template <int L, int R, int Ns, int S, unsigned ZIN = 0u, unsigned WOUT = ~0u, class V>
 __device__ __forceinline__ void st_stage(const V* in, V* out, const V* tw, int lane) {
 #pragma unroll
-  for (int j = lane; j < L / R; j += 64) st_stage1<L, R, Ns, S>(in, out, tw, j);
+  for (int j = lane; j < L / R; j += 64) st_stage1<L, R, Ns, S, V, ZIN, WOUT>(in, out, tw, j);
 }
+
+// Rows of a line that the padded streamwise transforms never need, for a
+// first inverse (last forward) radix-R stage with Ns = 1 (= L/R): row r holds
+// the points r L/R .. r L/R + L/R - 1.  With Mx = L/3 kept modes the band
+// Mx .. L - Mx of the inverse input is zero (the 3/2 padding), and the same
+// band of the forward output is discarded (truncation to Mx modes).
+template <int L, int R>
+struct PadRows {
+  static constexpr int LR = L / R, MX = L / 3;
+  static constexpr int lo = (MX + LR - 1) / LR;           // first row wholly inside the band
+  static constexpr int hi = (L - MX + 1) / LR - 1;        // last such row
+  static constexpr unsigned band = hi >= lo ? (((1u << (hi + 1)) - 1u) & ~((1u << lo) - 1u)) : 0u;
+  static constexpr unsigned below = (1u << lo) - 1u;      // rows holding modes < Mx only partly or wholly
+};
 
 // Orders the LDS accesses of one wavefront: its lanes see each other's earlier
 // writes after this point (a wavefront's LDS instructions execute in order;
@@ -617,9 +642,12 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
         }
       }
     }
-    for (int m = MX + lane; m <= L - MX; m += 64) b0[spad(m)] = C2<T>::make(0, 0);
+    // the band rows PadRows::band are never read; the band points outside them are
+    using PR = PadRows<L, RA>;
+    for (int m = MX + lane; m < PR::lo * PR::LR; m += 64) b0[spad(m)] = C2<T>::make(0, 0);
+    for (int m = (PR::hi + 1) * PR::LR + lane; m <= L - MX; m += 64) b0[spad(m)] = C2<T>::make(0, 0);
     wave_sync();
-    st_stage<L, RA, 1, +1>(b0, b1, tw, lane);
+    st_stage<L, RA, 1, +1, PR::band>(b0, b1, tw, lane);
     wave_sync();
     st_stage<L, RB, RA, +1>(b1, b0, tw, lane);
     wave_sync();
@@ -660,7 +688,13 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
     wave_sync();
     st_stage<L, RB, RC, -1>(b0, b1, tw, lane);
     wave_sync();
-    st_stage<L, RA, RC * RB, -1>(b1, b0, tw, lane);
+    // last stage: the band rows are discarded; the L_z transform (q = 1)
+    // keeps modes below Mx only
+    using PR = PadRows<L, RA>;
+    if (q == 0)
+      st_stage<L, RA, RC * RB, -1, 0u, ~PR::band>(b1, b0, tw, lane);
+    else
+      st_stage<L, RA, RC * RB, -1, 0u, PR::below>(b1, b0, tw, lane);
     wave_sync();
     if (q == 0) {
       // separate the packed pair: F1 = (G(m) + conj G(-m))/2, F2 = (G(m) - conj G(-m))/(2i)
