@@ -70,7 +70,6 @@ def kernel_work(d, precision="fp64"):
         "analysis": (3 * (Y + S), 2.0 * Ny * Ph * col * 3),
         "hassemble": (5 * S + 4 * Ny * 16, 0.0),   # plus the mean-mode point block
         "synth_points": (4 * S, 2.0 * Pn * Ny * col * 2),
-        "rhs_prep": (4 * S, 0.0),
         "rhs_v": (S + 2 * (Ny - 4) / Ny * S + rv_rows / Ny * S,
                   2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
         "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S,
@@ -104,7 +103,7 @@ def roofline_of(name, avg_ms, work, precision="fp64"):
 KERNEL_SYMBOL = {"synth_padded": "k_mfma_cos", "zfft_inv": "k_zfft", "xpass": "k_xpass_st",
                  "zfft_fwd": "k_zfft", "analysis": "k_mfma_cos", "hassemble": "k_hassemble",
                  "synth_points": "k_mfma_cos",
-                 "rhs_prep": "k_rhs_prep", "rhs_v": "k_mfma_cos", "rhs_o": "k_mfma_cos",
+                 "rhs_v": "k_mfma_cos", "rhs_o": "k_mfma_cos",
                  "solve": "k_solve", "kinematics": "k_kinematics", "wsolve": "k_wsolve"}
 
 
@@ -167,14 +166,22 @@ def run_ensemble(steps, warmup):
         t_eager = time.perf_counter() - t0
         finite = all(np.all(np.isfinite(ens.member(i).get_state()["vnp1"])) for i in range(len(members)))
     sizes = {}
+    step_bytes = 0.0
     for m in members:
         key = f"{m['nnx']}x{m['nnz'] + 1}x{m['nny']}"
         sizes[key] = sizes.get(key, 0) + 1
+        work = kernel_work(dims(m), m.get("precision", "mixed"))
+        step_bytes += sum(nbytes for nbytes, _ in work.values())
+    agg_gbs = step_bytes / (t_graph / steps) / 1e9
     out = {
         "metric": "ensemble timesteps/s (24 channels advanced together, one hipGraph launch per step)",
         "value": round(steps / t_graph, 3), "unit": "ensemble steps/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": round(t_graph * 1e3 / steps, 4), "higher_is_better": True,
         "member_steps_per_s": round(len(members) * steps / t_graph, 1),
+        # algorithmic bytes of one step of every member (each kernel class reading
+        # its inputs and writing its outputs once) over the measured step time
+        "aggregate_hbm": {"achieved": round(agg_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(agg_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_step": int(step_bytes)},
         "eager_streams_ms_per_step": round(t_eager * 1e3 / steps, 4),
         "dtype": "f32 transforms and products, f64 solves and state", "data": "synthetic",
         "config": {"workload": "ensemble24: re_c in [2000, 6000], dt=1e-3, mixed precision",
