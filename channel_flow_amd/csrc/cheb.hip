@@ -133,12 +133,16 @@ hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const
 // ----------------------------------------------------------------- analysis
 // coefficient k = 2j+m: (1/(2N c_k)) sum_p w_p cos(pi k p/2N) (x_p +- x_{2N-p})
 // over the folded points p = 0..N (PF:1686-1700: /2N, c_0 = 2, w_0 = 1, else 2).
+// For N even the last even coefficient, k = N, has the weights cos(pi p/2) =
+// 1, 0, -1, 0, ... : it would be a lone row of a 16-row block (N = 192: 97
+// even rows = 6 blocks + 1), so it is summed outside the matrix cores, as the
+// synthesis sums its centre point.
 template <class TI, class TO, class TCc>
 struct AnalP {
   using TC = TCc;
   static constexpr int NACC = 2, NB = 2, KC = 16;  // KC: k rows per B tile
   static constexpr bool kRecurrence = true;
-  static constexpr bool kCentre = false;
+  static constexpr bool kCentre = true;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
   static constexpr int kMinWaves = 1;
   static constexpr int bsel(int m) { return m; }
@@ -151,6 +155,7 @@ struct AnalP {
   const TI* in;
   TO* out;
   int P, N;
+  int centre;    // N even: coefficient k = N summed outside the matrix cores
   long ncold;
   double inv2n;
   __device__ void select(int z) {
@@ -160,6 +165,9 @@ struct AnalP {
   __device__ int aindex(int m, int row, int k0, int n4_) const {
     return (int)((unsigned)((2 * row + m) * k0) % (unsigned)n4_);
   }
+  // weights cos(pi N p / 2N) = cos(pi p / 2) of the even stream, p = k mod 4
+  static __device__ TC centre_sign(int kk) { return (kk & 1) ? (TC)0 : ((kk & 2) ? (TC)-1 : (TC)1); }
+  __device__ void store_centre(long col, TC v) const { out[(long)N * ncold + col] = (TO)((double)v * inv2n); }
   __device__ int astep(int m, int row, int n4_) const { return (int)((unsigned)(4 * (2 * row + m)) % (unsigned)n4_); }
   // B streams: weighted folded sums w_p (x_p +- x_{P-1-p}); the loads are
   // issued ahead, the folding is done when the tile is staged
@@ -197,7 +205,8 @@ static hipError_t analysis_t(Ctx& c, const TC* costab, const TI* const* in, doub
   pb.costab = costab;
   pb.rtab = c.costab;
   pb.n4 = 4 * c.d.N;
-  pb.sh = mfma_shape(c.d.ne, c.d.Ph);
+  pb.centre = (c.d.N % 2 == 0) ? 1 : 0;
+  pb.sh = mfma_shape(c.d.ne - pb.centre, c.d.Ph);
   for (int i = 0; i < nf; ++i) {
     pb.inf[i] = in[i];
     pb.outf[i] = reinterpret_cast<double*>(out[i]);
