@@ -660,6 +660,23 @@ int solve(Ctx* c) {
   return 0;
 }
 
+// solve() split around k_solve, for an ensemble, whose members' k_solve blocks
+// run as one launch: the right-hand sides and the diagonalised omega_y solve
+// before it (the omega_y path shares no data with k_solve), v's transpose and
+// the kinematics after it.  Each kernel computes what it computes in solve().
+int solve_before(Ctx* c) {
+  CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
+  CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
+  if (c->wdiag) CHF_TRY(timed(c, K_WSOLVE, [&] { return launch_wsolve(*c); }));
+  return 0;
+}
+
+int solve_after(Ctx* c) {
+  CHF_TRY(launch_vtranspose(*c));
+  CHF_TRY(timed(c, K_KINEMATICS, [&] { return launch_kinematics(*c); }));
+  return 0;
+}
+
 // Waits for the context's work and reports a state that the last kinematics
 // pass found to hold a NaN or an infinity (v, omega_y, U or W).
 int check_finite(Ctx* c, const char* who) {
@@ -980,21 +997,30 @@ int chf_timing(void* ctx, int id, double* total_ms, int* launches) {
 // ------------------------------------------------------------------ ensemble
 //
 // Independent channels of any mix of sizes, Reynolds numbers and precisions
-// advanced together.  One step of all members is one hipGraph launch: the
-// members' kernel sequences [solve; nonlinear] are captured as parallel
+// advanced together.  One step of all members is one hipGraph launch of three
+// stages: every member's right-hand sides and omega_y solve as parallel
 // branches (each member on its own stream, forked from and joined to the
-// ensemble stream), so small members fill the GPU together.  The AB2 history
-// buffers swap every step, so a graph is captured per combination of the
+// ensemble stream); one k_solve_members launch that streams every member's
+// half-system inverses (PF:242-394 over all members); then every member's
+// kinematics and nonlinear terms as parallel branches again.  Small members
+// fill the GPU together.  The AB2 history buffers swap every step, so a graph
+// (with the solve arguments it names) is captured per combination of the
 // members' history phases (normally two) and cached.
 
 namespace {
 
+struct EnsGraph {
+  std::vector<int> phase;         // members' history phases the graph was captured at
+  hipGraphExec_t exec = nullptr;
+  void* solve_args = nullptr;     // device: the members' k_solve arguments for that phase
+};
+
 struct Ens {
   std::vector<Ctx*> m;
   hipStream_t origin = nullptr;
-  hipEvent_t fork = nullptr;
-  std::vector<hipEvent_t> join;
-  std::vector<std::pair<std::vector<int>, hipGraphExec_t>> graphs;
+  hipEvent_t fork = nullptr, mid = nullptr;
+  std::vector<hipEvent_t> ready, join;
+  std::vector<EnsGraph> graphs;
   std::string err;
 };
 
@@ -1010,7 +1036,10 @@ int ens_fail(Ens* e, const std::string& msg, int rc = -1) {
   } while (0)
 
 void ens_free(Ens* e) {
-  for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : e->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.solve_args) (void)hipFree(g.solve_args);
+  }
   e->graphs.clear();
   for (Ctx* c : e->m) {
     if (!c) continue;
@@ -1021,24 +1050,44 @@ void ens_free(Ens* e) {
   e->m.clear();
   for (hipEvent_t ev : e->join) (void)hipEventDestroy(ev);
   e->join.clear();
+  for (hipEvent_t ev : e->ready) (void)hipEventDestroy(ev);
+  e->ready.clear();
+  if (e->mid) (void)hipEventDestroy(e->mid);
   if (e->fork) (void)hipEventDestroy(e->fork);
   if (e->origin) (void)hipStreamDestroy(e->origin);
 }
 
-// Captures one step of every member; member host state (history phase) is
-// restored afterwards, the caller advances it per launch.
-int ens_capture(Ens* e, hipGraphExec_t* out) {
-  std::vector<int> phase;
-  for (Ctx* c : e->m) phase.push_back(c->hcur);
+// Captures one step of every member into g (solve arguments written first,
+// outside the capture); member host state (history phase) is restored
+// afterwards, the caller advances it per launch.
+int ens_capture(Ens* e, EnsGraph* g) {
+  const int n = (int)e->m.size();
+  for (Ctx* c : e->m) g->phase.push_back(c->hcur);
+  ENS_TRY(hipMalloc(&g->solve_args, solve_members_bytes(n)));
+  ENS_TRY(write_solve_members(e->m.data(), n, g->solve_args));
   ENS_TRY(hipStreamBeginCapture(e->origin, hipStreamCaptureModeRelaxed));
   int rc = 0;
   std::string what;
   hipError_t he = hipEventRecord(e->fork, e->origin);
-  for (size_t i = 0; i < e->m.size() && he == hipSuccess && rc == 0; ++i) {
+  for (int i = 0; i < n && he == hipSuccess && rc == 0; ++i) {
     Ctx* c = e->m[i];
     he = hipStreamWaitEvent(c->stream, e->fork, 0);
     if (he != hipSuccess) break;
-    rc = solve(c);
+    rc = solve_before(c);
+    if (rc) {
+      what = c->err;
+      break;
+    }
+    he = hipEventRecord(e->ready[i], c->stream);
+    if (he == hipSuccess) he = hipStreamWaitEvent(e->origin, e->ready[i], 0);
+  }
+  if (he == hipSuccess && rc == 0) he = launch_solve_members(e->origin, e->m.data(), n, g->solve_args);
+  if (he == hipSuccess && rc == 0) he = hipEventRecord(e->mid, e->origin);
+  for (int i = 0; i < n && he == hipSuccess && rc == 0; ++i) {
+    Ctx* c = e->m[i];
+    he = hipStreamWaitEvent(c->stream, e->mid, 0);
+    if (he != hipSuccess) break;
+    rc = solve_after(c);
     if (rc == 0) rc = nonlinear(c);
     if (rc) {
       what = c->err;
@@ -1049,7 +1098,7 @@ int ens_capture(Ens* e, hipGraphExec_t* out) {
   }
   hipGraph_t graph = nullptr;
   const hipError_t ee = hipStreamEndCapture(e->origin, &graph);
-  for (size_t i = 0; i < e->m.size(); ++i) e->m[i]->hcur = phase[i];
+  for (int i = 0; i < n; ++i) e->m[i]->hcur = g->phase[i];
   if (rc) {
     if (graph) (void)hipGraphDestroy(graph);
     return ens_fail(e, "capture of a member step failed: " + what, rc);
@@ -1063,7 +1112,7 @@ int ens_capture(Ens* e, hipGraphExec_t* out) {
   const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   ENS_TRY(ie);
-  *out = exec;
+  g->exec = exec;
   return 0;
 }
 
@@ -1103,7 +1152,10 @@ int chf_ensemble_create(const chf_params* members, int n, void** out) {
   }
   hipError_t he = hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking);
   if (he == hipSuccess) he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&e->mid, hipEventDisableTiming);
+  e->ready.assign(n, nullptr);
   e->join.assign(n, nullptr);
+  for (int i = 0; i < n && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&e->ready[i], hipEventDisableTiming);
   for (int i = 0; i < n && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming);
   if (he != hipSuccess) {
     g_create_err = std::string("ensemble streams/events: ") + hipGetErrorString(he);
@@ -1156,11 +1208,17 @@ int chf_ensemble_step(void* ens, int n) {
     for (Ctx* c : e->m) phase.push_back(c->hcur);
     hipGraphExec_t exec = nullptr;
     for (auto& g : e->graphs)
-      if (g.first == phase) exec = g.second;
+      if (g.phase == phase) exec = g.exec;
     if (!exec) {
-      const int rc = ens_capture(e, &exec);
-      if (rc) return rc;
-      e->graphs.emplace_back(phase, exec);
+      EnsGraph g;
+      const int rc = ens_capture(e, &g);
+      if (rc) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.solve_args) (void)hipFree(g.solve_args);
+        return rc;
+      }
+      exec = g.exec;
+      e->graphs.push_back(g);
     }
     ENS_TRY(hipGraphLaunch(exec, e->origin));
     for (Ctx* c : e->m) c->hcur = 1 - c->hcur;  // what nonlinear() does per step

@@ -213,7 +213,14 @@ hipError_t launch_stream_c2r_unpadded(Ctx& c, const double2* in, double* out);
 hipError_t launch_physical(Ctx& c, const double2* field, double2* work, double* out);
 hipError_t launch_plane_stats(Ctx& c, const double* u, const double* v, const double* w, double* out);
 hipError_t launch_rhs(Ctx& c, int which);  // 0 = v, 1 = omega_y
-hipError_t launch_solve(Ctx& c);       // v, omega_y and the mean mode
+hipError_t launch_solve(Ctx& c);       // v, omega_y and the mean mode (k_solve, then launch_vtranspose)
+hipError_t launch_vtranspose(Ctx& c);  // v from the solve's per-group records to [Ny][Mz][Mx]
+// The k_solve blocks of several channels in one launch: solve_members_bytes(n)
+// of device memory hold their arguments, written by write_solve_members
+// (synchronous; the arguments name the members' current AB2 history phase).
+size_t solve_members_bytes(int n);
+hipError_t write_solve_members(Ctx* const* m, int n, void* dev);
+hipError_t launch_solve_members(hipStream_t st, Ctx* const* m, int n, const void* dev);
 hipError_t launch_wsolve(Ctx& c);      // diagonalised omega_y solve (all modes but the mean)
 hipError_t launch_kinematics(Ctx& c);  // u, w, omega_x, omega_z (and the next RHS brackets) from v, omega_y, U, W
 hipError_t launch_setup_inverses(Ctx& c);

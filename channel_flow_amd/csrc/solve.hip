@@ -219,11 +219,25 @@ __device__ __forceinline__ double2 stream_load(const double* p) {
 // beats 4 (8 loads, 3 waves), 5 and 8.
 constexpr int kSolveSU = 6;
 
-__global__ __launch_bounds__(512) void k_solve(SolveArgs a) {
+// MEMBERS: the solve of several independent channels (an ensemble) in one
+// launch.  Channel m owns the solve blocks [first[m], first[m + 1]) of the
+// grid, in the same XCD-contiguous order as a launch of its own; the block
+// size and LDS are the largest any channel needs.  Each block runs its
+// channel's block exactly as a launch of that channel alone would, so the
+// results are bit-identical to separate launches.
+template <bool MEMBERS>
+__global__ __launch_bounds__(512) void k_solve(SolveArgs a1, const SolveArgs* __restrict__ args,
+                                               const int* __restrict__ first, int nm) {
   extern __shared__ double rb[];  // [slot][Ny][4 vec], slot = group * neq + eq
   constexpr int SU = kSolveSU;
+  int b = xcd_group(blockIdx.x, gridDim.x);
+  int m = 0;
+  if constexpr (MEMBERS) {
+    while (m + 1 < nm && b >= first[m + 1]) ++m;
+    b -= first[m];
+  }
+  const SolveArgs& a = MEMBERS ? args[m] : a1;
   const int Ny = a.Ny, N = a.N, ne = a.ne, no = a.no;
-  const int b = xcd_group(blockIdx.x, gridDim.x);
   const int mat = a.bmat[b];
   const int gs0 = a.bg0[b], gs1 = a.bg1[b];
   const int ngr = gs1 >= 0 ? 2 : 1;
@@ -596,7 +610,7 @@ __global__ __launch_bounds__(256) void k_vtranspose(const double2* __restrict__ 
   }
 }
 
-hipError_t launch_solve(Ctx& c) {
+static SolveArgs solve_args(const Ctx& c) {
   const Dims& d = c.d;
   SolveArgs a{};
   a.rvT = c.rvT;
@@ -632,25 +646,80 @@ hipError_t launch_solve(Ctx& c) {
   a.ncol = d.ncol;
   a.dt = c.prm.dt;
   a.re = c.prm.re;
-  // two groups per block, one or two equations each
-  const size_t shm = sizeof(double) * 4ull * d.Ny * 2 * (c.wdiag ? 1 : 2);
-  if (shm > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    if (e != hipSuccess) return e;
-  }
-  // one thread per task: with the omega_y operator diagonalised a block streams
-  // the two v half systems only (ne/2 + no/2 row pairs, 128 threads at N = 192),
-  // otherwise the omega_y half systems as well
+  return a;
+}
+
+// LDS bytes (two groups per block, one or two equations each) and threads of
+// one channel's solve blocks: one thread per task; with the omega_y operator
+// diagonalised a block streams the two v half systems only (ne/2 + no/2 row
+// pairs, 128 threads at N = 192), otherwise the omega_y half systems as well.
+static void solve_shape(const Ctx& c, size_t* shm, int* threads) {
+  const Dims& d = c.d;
+  *shm = sizeof(double) * 4ull * d.Ny * 2 * (c.wdiag ? 1 : 2);
   const int pairs = (d.ne + (d.ne & 1)) / 2 + (d.no + (d.no & 1)) / 2;
-  const int threads = ((c.wdiag ? 1 : 2) * pairs + 63) / 64 * 64;
-  if (threads > 512) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_solve, dim3(d.nblocks), dim3(threads), shm, c.stream, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  *threads = ((c.wdiag ? 1 : 2) * pairs + 63) / 64 * 64;
+}
+
+static hipError_t allow_shm(const void* kernel, size_t shm) {
+  if (shm <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+}
+
+hipError_t launch_vtranspose(Ctx& c) {
+  const Dims& d = c.d;
   const size_t tshm = sizeof(double2) * 2ull * kVL * d.Mx;
   if (tshm > 65536) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_vtranspose, dim3(d.Mz / 2, (d.Ny + kVL - 1) / kVL), dim3(256), tshm, c.stream,
                      (const double2*)c.vst, c.v, d.Mx, d.Mz, d.Ny, d.ncol);
+  return hipGetLastError();
+}
+
+hipError_t launch_solve(Ctx& c) {
+  const SolveArgs a = solve_args(c);
+  size_t shm;
+  int threads;
+  solve_shape(c, &shm, &threads);
+  if (threads > 512) return hipErrorInvalidValue;
+  hipError_t e = allow_shm((const void*)k_solve<false>, shm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_solve<false>, dim3(c.d.nblocks), dim3(threads), shm, c.stream, a, nullptr, nullptr, 1);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_vtranspose(c);
+}
+
+size_t solve_members_bytes(int n) { return sizeof(SolveArgs) * (size_t)n + sizeof(int) * (size_t)(n + 1); }
+
+hipError_t write_solve_members(Ctx* const* m, int n, void* dev) {
+  std::vector<char> h(solve_members_bytes(n));
+  SolveArgs* args = reinterpret_cast<SolveArgs*>(h.data());
+  int* first = reinterpret_cast<int*>(h.data() + sizeof(SolveArgs) * (size_t)n);
+  first[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    args[i] = solve_args(*m[i]);
+    first[i + 1] = first[i] + m[i]->d.nblocks;
+  }
+  return hipMemcpy(dev, h.data(), h.size(), hipMemcpyHostToDevice);
+}
+
+hipError_t launch_solve_members(hipStream_t st, Ctx* const* m, int n, const void* dev) {
+  size_t shm = 0;
+  int threads = 0;
+  long blocks = 0;
+  for (int i = 0; i < n; ++i) {
+    size_t s;
+    int t;
+    solve_shape(*m[i], &s, &t);
+    shm = s > shm ? s : shm;
+    threads = t > threads ? t : threads;
+    blocks += m[i]->d.nblocks;
+  }
+  if (threads > 512 || blocks > 0x7fffffffL) return hipErrorInvalidValue;
+  hipError_t e = allow_shm((const void*)k_solve<true>, shm);
+  if (e != hipSuccess) return e;
+  const SolveArgs* args = static_cast<const SolveArgs*>(dev);
+  const int* first = reinterpret_cast<const int*>(static_cast<const char*>(dev) + sizeof(SolveArgs) * (size_t)n);
+  hipLaunchKernelGGL(k_solve<true>, dim3((unsigned)blocks), dim3(threads), shm, st, SolveArgs{}, args, first, n);
   return hipGetLastError();
 }
 
