@@ -517,7 +517,9 @@ __global__ __launch_bounds__((NG * Geo<N1, N2>::G)) void k_xpass_reg(RegX<V> a) 
 
 // LDS slot of complex element a in a Stockham buffer: one spare slot every 8
 // elements spreads the stride-R writes of the early stages over the banks
-// (measured: every 8 beats every 16 and 32; no padding is 27% slower).
+// (measured: every 8 beats every 16 and 32; no padding is 27% slower; per
+// buffer hand-off, dropping it where both sides are contiguous, or swizzling
+// with XOR instead, measured no faster at L = 384).
 __device__ __forceinline__ int spad(int a) { return a + (a >> 3); }
 
 // One Stockham autosort radix-R stage over a line of L points held in LDS
@@ -561,6 +563,55 @@ template <int L, int R, int Ns, int S, unsigned ZIN = 0u, unsigned WOUT = ~0u, c
 __device__ __forceinline__ void st_stage(const V* in, V* out, const V* tw, int lane) {
 #pragma unroll
   for (int j = lane; j < L / R; j += 64) st_stage1<L, R, Ns, S, V, ZIN, WOUT>(in, out, tw, j);
+}
+
+// The twiddles W_{Ns R}^{r (j mod Ns)} (conjugated for S > 0), r = 1 .. R-1, of
+// butterfly j = lane of a stage with L / R <= 64 butterflies: the same for
+// every transform of the line, so they are read from the LDS table once per
+// line (lanes past L / R read valid entries they never use).
+template <int L, int R, int Ns, int S, class V>
+struct StTw {
+  V w[R - 1];
+  __device__ __forceinline__ void load(const V* tw, int lane) {
+    const int k = lane % Ns;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      V t = tw[r * k * (L / (Ns * R))];
+      if (S > 0) t.y = -t.y;
+      w[r - 1] = t;
+    }
+  }
+};
+
+// st_stage with the twiddles of this lane's butterfly in registers (L / R <= 64).
+template <int L, int R, int Ns, int S, unsigned ZIN = 0u, unsigned WOUT = ~0u, class V>
+__device__ __forceinline__ void st_stage_r(const V* in, V* out, const StTw<L, R, Ns, S, V>& tw, int lane) {
+  static_assert(L / R <= 64, "one butterfly per lane");
+  const int j = lane;
+  if (j >= L / R) return;
+  V v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if ((ZIN >> r) & 1u) {
+      v[r].x = 0;
+      v[r].y = 0;
+    } else {
+      v[r] = in[spad(j + r * (L / R))];
+    }
+  }
+  if (Ns > 1) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      if ((ZIN >> r) & 1u) continue;
+      v[r] = cmulv(v[r], tw.w[r - 1]);
+    }
+  }
+  Dft<R, S, V>::run(v);
+  const int k = j % Ns;
+  const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if ((WOUT >> r) & 1u) out[spad(base + r * Ns)] = v[r];
 }
 
 // Rows of a line that the padded streamwise transforms never need, for a
@@ -624,6 +675,28 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
   for (int i = threadIdx.x; i < L; i += 64 * NL) tw[i] = a.tw[i];
   __syncthreads();  // the only block-wide barrier: twiddles
   if (!valid) return;
+  // with one butterfly per lane in the LDS stages, this lane's twiddles of
+  // every stage are read once for the line's five transforms
+  // (only where the registers cost no occupancy: 16-byte elements at L = 384,
+  // 0.800 -> 0.743 ms; with 8-byte elements the occupancy they cost is worth
+  // more than the reads they save, 0.344 -> 0.367 ms, and at L = 192 they halve
+  // the 16-byte kernel's occupancy)
+  constexpr bool kRegTw = sizeof(V) == 16 && L == 384 && L / RA <= 64 && L / RB <= 64;
+  StTw<L, RB, RA, +1, V> twi2;            // inverse stage 2
+  StTw<L, RB, RC, -1, V> twf2;            // forward stage 2
+  StTw<L, RA, RC * RB, -1, V> twf3;       // forward stage 3
+  V twl[RC - 1];                          // last inverse stage (Ns = 64): conj W_L^(r lane)
+  if constexpr (kRegTw) {
+    twi2.load(tw, lane);
+    twf2.load(tw, lane);
+    twf3.load(tw, lane);
+#pragma unroll
+    for (int rr = 1; rr < RC; ++rr) {
+      V w = tw[rr * lane];
+      w.y = -w.y;
+      twl[rr - 1] = w;
+    }
+  }
   V r[3][RC];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -649,7 +722,10 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
     wave_sync();
     st_stage<L, RA, 1, +1, PR::band>(b0, b1, tw, lane);
     wave_sync();
-    st_stage<L, RB, RA, +1>(b1, b0, tw, lane);
+    if constexpr (kRegTw)
+      st_stage_r<L, RB, RA, +1>(b1, b0, twi2, lane);
+    else
+      st_stage<L, RB, RA, +1>(b1, b0, tw, lane);
     wave_sync();
     {
       // last inverse stage (Ns = 64): lane j reads j + 64 r, keeps the output in registers
@@ -657,8 +733,13 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
       for (int rr = 0; rr < RC; ++rr) r[q][rr] = b0[spad(lane + 64 * rr)];
 #pragma unroll
       for (int rr = 1; rr < RC; ++rr) {
-        V w = tw[rr * lane];
-        w.y = -w.y;
+        V w;
+        if constexpr (kRegTw) {
+          w = twl[rr - 1];
+        } else {
+          w = tw[rr * lane];
+          w.y = -w.y;
+        }
         r[q][rr] = cmulv(r[q][rr], w);
       }
       Dft<RC, +1, V>::run(r[q]);
@@ -686,15 +767,25 @@ __global__ __launch_bounds__(64 * NL) void k_xpass_st(RegX<V> a) {
 #pragma unroll
     for (int rr = 0; rr < RC; ++rr) b0[spad(lane * RC + rr)] = r[q][rr];
     wave_sync();
-    st_stage<L, RB, RC, -1>(b0, b1, tw, lane);
+    if constexpr (kRegTw)
+      st_stage_r<L, RB, RC, -1>(b0, b1, twf2, lane);
+    else
+      st_stage<L, RB, RC, -1>(b0, b1, tw, lane);
     wave_sync();
     // last stage: the band rows are discarded; the L_z transform (q = 1)
     // keeps modes below Mx only
     using PR = PadRows<L, RA>;
-    if (q == 0)
-      st_stage<L, RA, RC * RB, -1, 0u, ~PR::band>(b1, b0, tw, lane);
-    else
-      st_stage<L, RA, RC * RB, -1, 0u, PR::below>(b1, b0, tw, lane);
+    if constexpr (kRegTw) {
+      if (q == 0)
+        st_stage_r<L, RA, RC * RB, -1, 0u, ~PR::band>(b1, b0, twf3, lane);
+      else
+        st_stage_r<L, RA, RC * RB, -1, 0u, PR::below>(b1, b0, twf3, lane);
+    } else {
+      if (q == 0)
+        st_stage<L, RA, RC * RB, -1, 0u, ~PR::band>(b1, b0, tw, lane);
+      else
+        st_stage<L, RA, RC * RB, -1, 0u, PR::below>(b1, b0, tw, lane);
+    }
     wave_sync();
     if (q == 0) {
       // separate the packed pair: F1 = (G(m) + conj G(-m))/2, F2 = (G(m) - conj G(-m))/(2i)
