@@ -307,7 +307,7 @@ int autotune(Ctx* c) {
   st.push_back({K_SYNTH_POINTS, [c] {
                   const double2* hc[2] = {c->hvc, c->hgc};
                   double2* ho[2] = {c->hv[0], c->hg[0]};
-                  return launch_synth_points(*c, hc, ho, 2);
+                  return launch_synth_points_h(*c, hc, ho, 2);
                 }});
   st.push_back({K_RHS_V, [c] { return launch_rhs(*c, 0); }});
   st.push_back({K_RHS_O, [c] { return launch_rhs(*c, 1); }});
@@ -627,7 +627,7 @@ int nonlinear(Ctx* c) {
   const int nxt = 1 - c->hcur;
   const double2* hc[2] = {c->hvc, c->hgc};
   double2* ho[2] = {c->hv[nxt], c->hg[nxt]};
-  CHF_TRY(timed(c, K_SYNTH_POINTS, [&] { return launch_synth_points(*c, hc, ho, 2); }));
+  CHF_TRY(timed(c, K_SYNTH_POINTS, [&] { return launch_synth_points_h(*c, hc, ho, 2); }));
   c->hcur = nxt;
   return 0;
 }

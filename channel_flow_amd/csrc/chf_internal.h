@@ -207,6 +207,7 @@ hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const*
 hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_hassemble(Ctx& c);
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf);
+hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf);  // FP32 compute in the mixed form
 hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);
 hipError_t launch_stream_c2r_unpadded(Ctx& c, const double2* in, double* out);
 // unpadded physical field: work = complex [Ny][Mz][Mx], out = real [Ny][nny][nnx]
