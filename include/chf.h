@@ -157,7 +157,10 @@ int chf_set_tile_choice(void* ctx, int id, int cb);
 
 /* Ensemble: n independent channels (any mix of grid sizes, Reynolds numbers
  * and precisions) advanced together, one step of every member being one
- * hipGraph launch in which the members run as parallel branches.  The
+ * hipGraph launch: the members' right-hand sides as parallel branches, one
+ * solve launch over every member's solve blocks, then the members'
+ * kinematics and nonlinear terms as parallel branches (each member's result
+ * is bit-identical to the same channel stepped alone).  The
  * reference runs one channel per job, so these have no single counterpart;
  * each member is a full context whose state is exchanged with the calls
  * above (chf_set_state, chf_init_fields, chf_get_state, chf_get_fields), and
