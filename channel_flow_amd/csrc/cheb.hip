@@ -16,6 +16,8 @@
 //                  emitted as the parity-folded right-hand sides
 //   hassemble    : h_v = k^2 L_y + i alpha D L_x + i beta D L_z,
 //                  h_g = i alpha L_z - i beta L_x (PF:1502-1540, dz PF:570-599)
+#include <type_traits>
+
 #include "chf_internal.h"
 #include "mfma.h"
 
@@ -32,6 +34,7 @@ struct SynthP {
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = true;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
+  static constexpr bool kVecCols = sizeof(TO) == 4;  // FP32 rows: one vector store per lane
   // measured: FP32 five waves 0.372 -> 0.366 ms; FP64 six waves 0.640 -> 0.625 ms
   static constexpr int kMinWaves = sizeof(TCc) == 8 ? 6 : 5;
   static constexpr int bsel(int m) { return m; }
@@ -72,6 +75,22 @@ struct SynthP {
     out[r * ncold + col] = (TO)(acc[0] + acc[1]);
     const int q = P - 1 - r;
     if (q != r) out[q * ncold + col] = (TO)(acc[0] - acc[1]);
+  }
+  // CB adjacent columns from col (col a multiple of CB), as one vector store
+  template <int CB>
+  __device__ void store_vec(int r, long col, const TC (&acc)[2][CB]) const {
+    static_assert(CB == 2 || CB == 4, "vector stores of 2 or 4 values");
+    using VT = typename std::conditional<CB == 2, float2, float4>::type;
+    static_assert(sizeof(TO) == 4, "FP32 rows");
+    TO e[CB], o[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      e[c] = (TO)(acc[0][c] + acc[1][c]);
+      o[c] = (TO)(acc[0][c] - acc[1][c]);
+    }
+    *reinterpret_cast<VT*>(out + r * ncold + col) = *reinterpret_cast<const VT*>(e);
+    const int q = P - 1 - r;
+    if (q != r) *reinterpret_cast<VT*>(out + q * ncold + col) = *reinterpret_cast<const VT*>(o);
   }
   // y = 0: T_l(0) = (-1)^(l/2) for even l, 0 for odd l; even-stream row kk of
   // tile kc is l = 2 (KC kc + kk), KC even
@@ -159,6 +178,7 @@ struct AnalP {
   static constexpr bool kRecurrence = true;
   static constexpr bool kCentre = true;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
+  static constexpr bool kVecCols = false;
   static constexpr int kMinWaves = 1;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
@@ -274,6 +294,7 @@ struct RhsSynP {
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = false;
   static constexpr bool kWavePerRowBlock = true;  // 5-6 row blocks: one wave each (measured faster)
+  static constexpr bool kVecCols = false;
   static constexpr int kMinWaves = 1;
   static constexpr int bsel(int m) { return m; }
   const double* costab;

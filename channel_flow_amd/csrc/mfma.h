@@ -81,6 +81,14 @@ __device__ __forceinline__ T pick6(T const (&a)[6], int z) {
 // Minimum waves per SIMD the compiler must leave room for: P::kMinWaves for
 // tiles whose accumulators take at most 48 registers (wider tiles would
 // spill), otherwise no constraint.
+// Problems with P::kVecCols and tiles of CB >= 2 column blocks give each lane
+// CB adjacent columns (CB j .. CB j + CB-1 for lane j of a block row) instead
+// of columns j, j + 16, ...: the results are the same, and the epilogue writes
+// a lane's columns with one vector store, so each row of a tile leaves as
+// whole lines (FP32 rows of 16 columns are half a 128-byte line otherwise).
+template <class P, int CB>
+constexpr bool vec_cols() { return P::kVecCols && CB >= 2; }
+
 template <class P, int RBW, int CB>
 constexpr int min_waves() {
   return (P::NACC * RBW * CB * 4 * (int)sizeof(typename P::TC) / 4 <= 48) ? P::kMinWaves : 1;
@@ -194,7 +202,8 @@ void k_mfma_cos(P pb) {
 #pragma unroll
         for (int s = 0; s < NB; ++s)
 #pragma unroll
-          for (int c = 0; c < CB; ++c) bv[s][c] = sB[s][kb * 4 + (lane >> 4)][c * 16 + (lane & 15)];
+          for (int c = 0; c < CB; ++c)
+            bv[s][c] = sB[s][kb * 4 + (lane >> 4)][vec_cols<P, CB>() ? CB * (lane & 15) + c : c * 16 + (lane & 15)];
 #pragma unroll
         for (int r = 0; r < RBW; ++r) {
           if (FULL || wave + NWV * r < MB) {
@@ -226,6 +235,28 @@ void k_mfma_cos(P pb) {
       const long col = col0 + wave * 16 + lane;
       if (col < p.ncold) p.store_centre(col, cen);
     }
+  }
+  if constexpr (vec_cols<P, CB>()) {
+    // ncold is a multiple of 16 and so of CB: a lane's columns are all inside
+#pragma unroll
+    for (int r = 0; r < RBW; ++r) {
+      const int rb = wave + NWV * r;
+      if (!FULL && rb >= MB) continue;
+      const long col = col0 + CB * (lane & 15);
+      if (col >= p.ncold) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = rb * 16 + MF::row(lane, e);
+        if (row >= p.sh.M) continue;
+        TC v[NA][CB];
+#pragma unroll
+        for (int m = 0; m < NA; ++m)
+#pragma unroll
+          for (int c = 0; c < CB; ++c) v[m][c] = acc[m][r][c][e];
+        p.template store_vec<CB>(row, col, v);
+      }
+    }
+    return;
   }
 #pragma unroll
   for (int r = 0; r < RBW; ++r) {
