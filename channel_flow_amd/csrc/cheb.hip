@@ -23,6 +23,20 @@
 
 namespace chf {
 
+// CB consecutive values from dst (a multiple of CB elements from an aligned
+// base) as 8- or 16-byte vector stores.
+template <int CB, class T>
+__device__ __forceinline__ void store_row(T* dst, const T (&v)[CB]) {
+  static_assert(CB == 2 || CB == 4, "vector stores of 2 or 4 values");
+  if constexpr (sizeof(T) == 4) {
+    using VT = typename std::conditional<CB == 2, float2, float4>::type;
+    *reinterpret_cast<VT*>(dst) = *reinterpret_cast<const VT*>(v);
+  } else {
+#pragma unroll
+    for (int c = 0; c < CB; c += 2) *reinterpret_cast<double2*>(dst + c) = make_double2(v[c], v[c + 1]);
+  }
+}
+
 // ---------------------------------------------------------------- synthesis
 // out[p] = E + O, out[P-1-p] = E - O for p < Ph, E/O = even/odd degree sums,
 // with A_m(p, j) = T_{2j+m}(y_p) = cos(pi f p (2j+m) / 2N) (f = 1 on the
@@ -34,7 +48,7 @@ struct SynthP {
   static constexpr bool kRecurrence = false;
   static constexpr bool kCentre = true;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
-  static constexpr bool kVecCols = sizeof(TO) == 4;  // FP32 rows: one vector store per lane
+  static constexpr bool kVecCols = true;  // each lane's adjacent columns in vector stores
   // measured: FP32 five waves 0.372 -> 0.366 ms; FP64 six waves 0.640 -> 0.625 ms
   static constexpr int kMinWaves = sizeof(TCc) == 8 ? 6 : 5;
   static constexpr int bsel(int m) { return m; }
@@ -76,21 +90,18 @@ struct SynthP {
     const int q = P - 1 - r;
     if (q != r) out[q * ncold + col] = (TO)(acc[0] - acc[1]);
   }
-  // CB adjacent columns from col (col a multiple of CB), as one vector store
+  // CB adjacent columns from col (col a multiple of CB), as vector stores
   template <int CB>
   __device__ void store_vec(int r, long col, const TC (&acc)[2][CB]) const {
-    static_assert(CB == 2 || CB == 4, "vector stores of 2 or 4 values");
-    using VT = typename std::conditional<CB == 2, float2, float4>::type;
-    static_assert(sizeof(TO) == 4, "FP32 rows");
     TO e[CB], o[CB];
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       e[c] = (TO)(acc[0][c] + acc[1][c]);
       o[c] = (TO)(acc[0][c] - acc[1][c]);
     }
-    *reinterpret_cast<VT*>(out + r * ncold + col) = *reinterpret_cast<const VT*>(e);
+    store_row<CB>(out + r * ncold + col, e);
     const int q = P - 1 - r;
-    if (q != r) *reinterpret_cast<VT*>(out + q * ncold + col) = *reinterpret_cast<const VT*>(o);
+    if (q != r) store_row<CB>(out + q * ncold + col, o);
   }
   // y = 0: T_l(0) = (-1)^(l/2) for even l, 0 for odd l; even-stream row kk of
   // tile kc is l = 2 (KC kc + kk), KC even
@@ -178,7 +189,7 @@ struct AnalP {
   static constexpr bool kRecurrence = true;
   static constexpr bool kCentre = true;
   static constexpr bool kWavePerRowBlock = false;  // four-wave blocks (measured faster)
-  static constexpr bool kVecCols = false;
+  static constexpr bool kVecCols = true;  // each lane's adjacent columns in vector stores
   static constexpr int kMinWaves = 1;
   static constexpr int bsel(int m) { return m; }
   const TC* costab;
@@ -230,6 +241,18 @@ struct AnalP {
     const int k = 2 * r;
     out[k * ncold + col] = (TO)((double)acc[0] * (k == 0 ? 0.5 * inv2n : inv2n));
     if (k + 1 <= N) out[(k + 1) * ncold + col] = (TO)((double)acc[1] * inv2n);
+  }
+  template <int CB>
+  __device__ void store_vec(int r, long col, const TC (&acc)[2][CB]) const {
+    const int k = 2 * r;
+    TO e[CB], o[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      e[c] = (TO)((double)acc[0][c] * (k == 0 ? 0.5 * inv2n : inv2n));
+      o[c] = (TO)((double)acc[1][c] * inv2n);
+    }
+    store_row<CB>(out + k * ncold + col, e);
+    if (k + 1 <= N) store_row<CB>(out + (k + 1) * ncold + col, o);
   }
 };
 
