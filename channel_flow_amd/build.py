@@ -22,7 +22,7 @@ LIBDIR = os.path.join(PKG, "lib")
 OBJDIR = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(LIBDIR, "libchf.so")
 
-SOURCES = ["api.hip", "fft.hip", "cheb.hip", "solve.hip", "stats.hip"]
+SOURCES = ["api.hip", "fft.hip", "cheb.hip", "solve.hip", "stats.hip", "ydct.hip"]
 FORTRAN_DIR = os.path.join(PKG, "fortran")
 FORTRAN_SOURCES = ["chf_binding.f90", "paraandcons_gpu.f90", "poiseuilleflow_gpu.f90"]  # in module order
 DRIVER = os.path.join(LIBDIR, "poiseuilleflow_gpu")

@@ -157,7 +157,7 @@ void free_all(Ctx* c) {
   f(c->tsyn); f(c->tana); f(c->tpts); f(c->t0); f(c->t2); f(c->t4);
   f(c->inv); f(c->inv0); f(c->crep); f(c->bmat); f(c->bg0); f(c->bg1);
   for (int i = 0; i < 2; ++i) { f(c->wL[i]); f(c->wR[i]); f(c->wlam[i]); }
-  f(c->phys_c); f(c->phys_r[0]); f(c->phys_r[1]); f(c->phys_r[2]); f(c->stats); f(c->twxu); f(c->twzu); f(c->twx); f(c->twz); f(c->costab);
+  f(c->phys_c); f(c->phys_r[0]); f(c->phys_r[1]); f(c->phys_r[2]); f(c->stats); f(c->twxu); f(c->twzu); f(c->twx); f(c->twz); f(c->twy); f(c->costab);
   f(c->gv); f(c->go); f(c->nonfinite);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   c->event_pool.clear();
@@ -285,7 +285,10 @@ int autotune(Ctx* c) {
     std::function<hipError_t()> run;
   };
   std::vector<Stage> st;
-  if (c->prm.precision == 0) {
+  if (c->prm.precision == 0 && c->twy) {
+    // the padded synthesis, the analysis and the point values run as FFTs,
+    // which have no tile choice
+  } else if (c->prm.precision == 0) {
     st.push_back({K_SYNTH_PADDED, [c] {
                     const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
                     return launch_synth_padded(*c, in6, c->Y, 6);
@@ -304,11 +307,12 @@ int autotune(Ctx* c) {
                     return launch_analysis32(*c, y3, c->Lc, 3);
                   }});
   }
-  st.push_back({K_SYNTH_POINTS, [c] {
-                  const double2* hc[2] = {c->hvc, c->hgc};
-                  double2* ho[2] = {c->hv[0], c->hg[0]};
-                  return launch_synth_points_h(*c, hc, ho, 2);
-                }});
+  if (!(c->twy && c->prm.precision == 0))
+    st.push_back({K_SYNTH_POINTS, [c] {
+                    const double2* hc[2] = {c->hvc, c->hgc};
+                    double2* ho[2] = {c->hv[0], c->hg[0]};
+                    return launch_synth_points_h(*c, hc, ho, 2);
+                  }});
   st.push_back({K_RHS_V, [c] { return launch_rhs(*c, 0); }});
   st.push_back({K_RHS_O, [c] { return launch_rhs(*c, 1); }});
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -552,6 +556,21 @@ int build(Ctx* c) {
     CHF_TRY(dalloc(c, &c->twzf, d.Lz));
     CHF_TRY(hipMemcpy(c->twxf, txf.data(), d.Lx * sizeof(float2), hipMemcpyHostToDevice));
     CHF_TRY(hipMemcpy(c->twzf, tzf.data(), d.Lz * sizeof(float2), hipMemcpyHostToDevice));
+  }
+
+  // tables of the FFT forms of the FP64 wall-normal transforms (2N-point and
+  // N-point twiddles, the type-III DCT pre-twiddles e^{i pi k/2N} =
+  // cos(pi k/2N) + i cos(pi (N-k)/2N)); their blocks take 8 adjacent columns.
+  // CHF_Y_TRANSFORMS=matrix in the environment keeps the cos-table products.
+  const char* ytr = std::getenv("CHF_Y_TRANSFORMS");
+  const bool ymatrix = ytr && std::strcmp(ytr, "matrix") == 0;
+  if (!ymatrix && ydct_length_supported(d.N) && d.ncol % 8 == 0) {
+    const int N = d.N;
+    std::vector<double2> ty = twiddles(2 * N), tn = twiddles(N);
+    ty.insert(ty.end(), tn.begin(), tn.end());
+    for (int k = 0; k < N; ++k) ty.push_back(make_double2(cospi_frac(k, 2L * N), cospi_frac(N - k, 2L * N)));
+    CHF_TRY(dalloc(c, &c->twy, ty.size()));
+    CHF_TRY(hipMemcpy(c->twy, ty.data(), ty.size() * sizeof(double2), hipMemcpyHostToDevice));
   }
 
   // twiddles of the unpadded physical transform (lengths nnx and nny)

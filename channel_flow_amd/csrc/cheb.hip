@@ -135,7 +135,14 @@ static void as_real(const double2* const* in, const double** out, int nf) {
   for (int i = 0; i < nf; ++i) out[i] = reinterpret_cast<const double*>(in[i]);
 }
 
+// The FP64 entry points below run as FFTs along y (ydct.hip) when 2N has an
+// FFT split (c.twy set), as cos-table products on the FP64 matrix cores
+// otherwise.  At 256x193x256: padded synthesis 0.625 -> 0.463 ms, analysis
+// 0.365 -> 0.294 ms, point values 0.134 -> 0.090 ms.  FFT forms in FP32
+// arithmetic were measured too and kept out: padded synthesis 0.343 ms against
+// 0.35 on the FP32 matrix cores, analysis 0.323 against 0.20.
 hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const* out, int nf) {
+  if (c.twy) return launch_ysynth(c, in, out, nf);
   const double* ri[6];
   double* ro[6];
   as_real(in, ri, nf);
@@ -144,6 +151,7 @@ hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const*
 }
 
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf) {
+  if (c.twy) return launch_ypoints(c, in, out, nf);
   const double* ri[6];
   double* ro[6];
   as_real(in, ri, nf);
@@ -277,6 +285,7 @@ static hipError_t analysis_t(Ctx& c, const TC* costab, const TI* const* in, doub
 }
 
 hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
+  if (c.twy) return launch_yanalysis(c, in, out, nf);
   const double* ri[6];
   as_real(in, ri, nf);
   return analysis_t<double, double>(c, c.costab, ri, out, nf);

@@ -185,12 +185,14 @@ def test_mixed_form_against_fp64_and_oracle(require_gpu):
         assert e1[k] < 1e-10, (k, e1[k])
 
 
-@pytest.mark.parametrize("nnz", [96, 192, 256])
+@pytest.mark.parametrize("nnz", [48, 64, 96, 100, 128, 192, 256])
 def test_large_degree_small_box(require_gpu, nnz):
-    """N = 96 and 192 (the multi-row-block contraction paths used at the
-    256x193x256 workload) and 256 (the 512x257x512 degree, whose diagonalised
-    omega_y solve takes two row blocks per wave) on a 16x16 box: initial() + 3
-    steps vs the oracle."""
+    """initial() + 3 steps vs the oracle on a 16x16 box, FP64 form.  N = 48,
+    64, 96, 128, 192 and 256 run the wall-normal transforms as FFTs, one of
+    each 2N split (96 ... 512 points); N = 100 (2N = 200 has no split) runs
+    them as cos-table products on the matrix cores.  N = 192 is the
+    256x193x256 degree, N = 256 the 512x257x512 one (its diagonalised omega_y
+    solve takes two row blocks per wave)."""
     cfg = dict(nnx=16, nny=16, nnz=nnz, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
                dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=21)
     p = params_of(cfg)
@@ -273,6 +275,42 @@ def test_autotuner_reports_choices(require_gpu):
     for name, t in tiles.items():
         assert t["choice"] in (1, 2, 4), (name, t)
         assert all(ms > 0 for ms in t["ms"]), (name, t)
+
+
+def test_fft_forms_have_no_tile_choice(require_gpu):
+    """At N = 192 the FP64 form runs the padded synthesis, the analysis and the
+    point values as FFTs (no tile choice); the mixed form keeps all five
+    cos-table contractions on the FP32/FP64 matrix cores."""
+    cfg = dict(nnx=32, nny=32, nnz=192, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
+               dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=21)
+    with make_channel(cfg, "fp64") as ch:
+        assert set(ch.tile_choices()) == {"rhs_v", "rhs_o"}, ch.tile_choices()
+    with make_channel(cfg, "mixed") as ch:
+        assert set(ch.tile_choices()) == {"synth_padded", "analysis", "synth_points", "rhs_v", "rhs_o"}
+
+
+def test_fft_and_matrix_transforms_agree(require_gpu, monkeypatch):
+    """The FP64 form's wall-normal transforms as FFTs along y (the default at
+    N = 192) and as cos-table products on the FP64 matrix cores
+    (CHF_Y_TRANSFORMS=matrix) are two independent evaluations of the same
+    sums: after initial() and 3 steps every state array agrees to 1e-11."""
+    cfg = dict(nnx=32, nny=32, nnz=192, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
+               dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=25)
+    v, oy, au0, w0 = ic_of(cfg)
+    states = {}
+    for mode in ("fft", "matrix"):
+        if mode == "matrix":
+            monkeypatch.setenv("CHF_Y_TRANSFORMS", "matrix")
+        with make_channel(cfg) as ch:
+            assert ("synth_padded" in ch.tile_choices()) == (mode == "matrix"), mode
+            ch.initial(v, oy, au0, w0)
+            ch.step(3)
+            states[mode] = ch.get_state()
+        monkeypatch.delenv("CHF_Y_TRANSFORMS", raising=False)
+    e = state_errors(states["fft"], states["matrix"])
+    report("fft vs matrix transforms, 3 steps", e)
+    for k, err in e.items():
+        assert err < ONE_OP_TOL, (k, err)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
