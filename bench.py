@@ -10,10 +10,10 @@ when the timed region starts.  The default workload is the 256 x 193 x 256
 channel (nnx = nny = 256, N = 192, Lx = 2 pi, Lz = pi, Re_tau ~ 395 class, in
 centreline units: U_c = 1, re = 7940, dt = 2.5e-4) from a seeded
 divergence-free initial condition.  Prints one JSON line: `value` is the
-mixed form; `fp64_value` the FP64 form (its wall-normal transforms as FFTs
-along y), `fp64_matrix_value` the FP64 form with those transforms as cos-table
-products on the matrix cores; `mixed_over_fp64` and `mixed_over_fp64_matrix`
-are the ratios.
+mixed form and `fp64_value` the FP64 form (wall-normal transforms as FFTs
+along y in both); `mixed_matrix_value` and `fp64_matrix_value` are the two
+forms with those transforms as cos-table products on the matrix cores;
+`mixed_over_fp64` and `mixed_over_fp64_matrix` are the ratios.
 """
 from __future__ import annotations
 
@@ -55,9 +55,9 @@ def kernel_work(d, precision="fp64", yfft=False):
     complex value); the wall-normal/span point arrays of the nonlinear term are
     FP64 in the FP64 form and FP32 (8 B) in the mixed form.  Flops count the
     useful multiply-adds (2 flops each) of the parity-split contractions.
-    With yfft (the FP64 form's wall-normal transforms run as FFTs along y) the
-    padded synthesis, the analysis and the point values have O(N log N) work
-    per column and are priced by their bytes alone.
+    With yfft (the wall-normal transforms run as FFTs along y) the padded
+    synthesis and the analysis, and the FP64 form's point values, have
+    O(N log N) work per column and are priced by their bytes alone.
     """
     w = 16 if precision == "fp64" else 8
     S = d["Ny"] * d["ncol"] * 16            # spectral field (FP64)
@@ -70,6 +70,7 @@ def kernel_work(d, precision="fp64", yfft=False):
     inv_v_bytes = (d["nclass"] - 1) * (ne * ne + no * no) * 8   # one per k^2 class; k^2 = 0 has no v system
     ws_flops = 2.0 * col * ((ne - 1) * (ne - 1) + ne * (ne - 1) + (no - 1) * (no - 1) + no * (no - 1))
     yf = 0.0 if yfft else 1.0
+    yp = 0.0 if yfft and precision == "fp64" else 1.0
     return {
         "synth_padded": (6 * (S + Y), yf * 2.0 * Ph * Ny * col * 6),
         "zfft_inv": (6 * (Y + Z), 0.0),
@@ -77,7 +78,7 @@ def kernel_work(d, precision="fp64", yfft=False):
         "zfft_fwd": (3 * (Z + Y), 0.0),
         "analysis": (3 * (Y + S), yf * 2.0 * Ny * Ph * col * 3),
         "hassemble": (5 * S + 4 * Ny * 16, 0.0),   # plus the mean-mode point block
-        "synth_points": (4 * S, yf * 2.0 * Pn * Ny * col * 2),
+        "synth_points": (4 * S, yp * 2.0 * Pn * Ny * col * 2),
         "rhs_v": (S + 2 * (Ny - 4) / Ny * S + rv_rows / Ny * S,
                   2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
         "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S,
@@ -254,8 +255,8 @@ def run_form(cfg, ic_arrays, precision, steps, warmup, total, log, y_matrix=Fals
     """Set-up, warm-up, the timed region (wall clock, no per-kernel events),
     the same number of steps again with per-kernel HIP events (the launches
     then run one at a time), then untimed steps up to `total` for the error
-    report.  y_matrix: the FP64 wall-normal transforms as cos-table products
-    on the matrix cores (CHF_Y_TRANSFORMS=matrix) instead of FFTs."""
+    report.  y_matrix: the wall-normal transforms as cos-table products on
+    the matrix cores (CHF_Y_TRANSFORMS=matrix) instead of FFTs."""
     from channel_flow_amd.channel import Channel
     v, oy, au0, w0 = ic_arrays
     t0 = time.perf_counter()
@@ -288,8 +289,8 @@ def run_form(cfg, ic_arrays, precision, steps, warmup, total, log, y_matrix=Fals
     ch.close()
     per_kernel = {k: dict(total_ms=round(ms, 3), launches=n, avg_ms=round(ms / n, 4) if n else None)
                   for k, (ms, n) in kt.items() if n}
-    # the FP64 form's transforms ran as FFTs when they have no tile choice
-    yfft = precision == "fp64" and "synth_padded" not in tiles
+    # the transforms ran as FFTs when they have no tile choice
+    yfft = "synth_padded" not in tiles
     return dict(ms_per_step=wall * 1e3 / steps, per_kernel=per_kernel, state=st, fields=fl,
                 setup_s=t_setup, tiles=tiles, yfft=yfft)
 
@@ -364,10 +365,12 @@ def main(argv=None):
     forms = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     total = max(a.error_steps, a.warmup + 2 * a.steps) if a.precision == "both" else a.warmup + 2 * a.steps
     runs = {f: run_form(cfg, ic_arrays, f, a.steps, a.warmup, total, log) for f in forms}
-    if "fp64" in runs and runs["fp64"]["yfft"] and a.precision == "both":
-        # the FP64 form with its transforms on the matrix cores, for the ratio
-        runs["fp64_matrix"] = run_form(cfg, ic_arrays, "fp64", a.steps, a.warmup, a.warmup + 2 * a.steps, log,
-                                       y_matrix=True)
+    if a.precision == "both":
+        # both forms with their transforms on the matrix cores, for comparison
+        for f in forms:
+            if runs[f]["yfft"]:
+                runs[f + "_matrix"] = run_form(cfg, ic_arrays, f, a.steps, a.warmup, a.warmup + 2 * a.steps, log,
+                                               y_matrix=True)
     head = "mixed" if "mixed" in runs else "fp64"
     r = runs[head]
     finite = all(np.all(np.isfinite(r["state"][k])) for k in ("vnp1", "omegaynp1", "au0"))
@@ -377,7 +380,7 @@ def main(argv=None):
     roof = roofline_of(dom, pk[dom]["avg_ms"], work, head)
     roof["traffic"] = measured_traffic(a.config, head, dom)
     for f, rr in runs.items():
-        pf = "mixed" if f == "mixed" else "fp64"
+        pf = "mixed" if f.startswith("mixed") else "fp64"
         w = kernel_work(d, pf, rr["yfft"])
         log(f, json.dumps({k: roofline_of(k, rr["per_kernel"][k]["avg_ms"], w, pf)
                            for k in rr["per_kernel"]}, indent=1))
@@ -411,13 +414,15 @@ def main(argv=None):
         out["fp64_value"] = round(1e3 / runs["fp64"]["ms_per_step"], 3)
         out["mixed_over_fp64"] = round(runs["fp64"]["ms_per_step"] / runs["mixed"]["ms_per_step"], 3)
         out["fp64_kernels"] = runs["fp64"]["per_kernel"]
-        out["fp64_y_transforms"] = "fft" if runs["fp64"]["yfft"] else "matrix"
-        if "fp64_matrix" in runs:
-            mm = runs["fp64_matrix"]["ms_per_step"]
-            out["fp64_matrix_value"] = round(1e3 / mm, 3)
-            out["mixed_over_fp64_matrix"] = round(mm / runs["mixed"]["ms_per_step"], 3)
-            out["fp64_matrix_kernels"] = {k: runs["fp64_matrix"]["per_kernel"][k]
-                                          for k in ("synth_padded", "analysis", "synth_points")}
+        out["y_transforms"] = "fft" if runs["fp64"]["yfft"] else "matrix"
+        if "fp64_matrix" in runs and "mixed_matrix" in runs:
+            fm, mm = runs["fp64_matrix"]["ms_per_step"], runs["mixed_matrix"]["ms_per_step"]
+            out["fp64_matrix_value"] = round(1e3 / fm, 3)
+            out["mixed_matrix_value"] = round(1e3 / mm, 3)
+            out["mixed_over_fp64_matrix"] = round(fm / mm, 3)
+            out["matrix_kernels"] = {f: {k: runs[f + "_matrix"]["per_kernel"][k]["avg_ms"]
+                                         for k in ("synth_padded", "analysis", "synth_points")}
+                                     for f in ("fp64", "mixed")}
         out["errors_vs_fp64"] = dict(field_errors(cfg, runs["mixed"], runs["fp64"]), steps=total)
     print(json.dumps(out), flush=True)
     return 0

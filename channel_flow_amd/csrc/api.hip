@@ -285,9 +285,9 @@ int autotune(Ctx* c) {
     std::function<hipError_t()> run;
   };
   std::vector<Stage> st;
-  if (c->prm.precision == 0 && c->twy) {
-    // the padded synthesis, the analysis and the point values run as FFTs,
-    // which have no tile choice
+  if (c->twy) {
+    // the padded synthesis and the analysis (and the FP64 form's point
+    // values) run as FFTs, which have no tile choice
   } else if (c->prm.precision == 0) {
     st.push_back({K_SYNTH_PADDED, [c] {
                     const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
@@ -558,13 +558,14 @@ int build(Ctx* c) {
     CHF_TRY(hipMemcpy(c->twzf, tzf.data(), d.Lz * sizeof(float2), hipMemcpyHostToDevice));
   }
 
-  // tables of the FFT forms of the FP64 wall-normal transforms (2N-point and
+  // tables of the FFT forms of the wall-normal transforms (2N-point and
   // N-point twiddles, the type-III DCT pre-twiddles e^{i pi k/2N} =
-  // cos(pi k/2N) + i cos(pi (N-k)/2N)); their blocks take 8 adjacent columns.
-  // CHF_Y_TRANSFORMS=matrix in the environment keeps the cos-table products.
+  // cos(pi k/2N) + i cos(pi (N-k)/2N)); their blocks take 8 (FP64) or 16
+  // (FP32) adjacent columns.  CHF_Y_TRANSFORMS=matrix in the environment keeps
+  // the cos-table products.
   const char* ytr = std::getenv("CHF_Y_TRANSFORMS");
   const bool ymatrix = ytr && std::strcmp(ytr, "matrix") == 0;
-  if (!ymatrix && ydct_length_supported(d.N) && d.ncol % 8 == 0) {
+  if (!ymatrix && ydct_length_supported(d.N) && d.ncol % 16 == 0) {
     const int N = d.N;
     std::vector<double2> ty = twiddles(2 * N), tn = twiddles(N);
     ty.insert(ty.end(), tn.begin(), tn.end());

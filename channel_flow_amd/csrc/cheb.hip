@@ -135,12 +135,13 @@ static void as_real(const double2* const* in, const double** out, int nf) {
   for (int i = 0; i < nf; ++i) out[i] = reinterpret_cast<const double*>(in[i]);
 }
 
-// The FP64 entry points below run as FFTs along y (ydct.hip) when 2N has an
-// FFT split (c.twy set), as cos-table products on the FP64 matrix cores
-// otherwise.  At 256x193x256: padded synthesis 0.625 -> 0.463 ms, analysis
-// 0.365 -> 0.294 ms, point values 0.134 -> 0.090 ms.  FFT forms in FP32
-// arithmetic were measured too and kept out: padded synthesis 0.343 ms against
-// 0.35 on the FP32 matrix cores, analysis 0.323 against 0.20.
+// The padded synthesis, the analysis and the FP64 point values below run as
+// FFTs along y (ydct.hip) when 2N has an FFT split (c.twy set), as cos-table
+// products on the matrix cores otherwise.  At 256x193x256, FP64: padded
+// synthesis 0.625 -> 0.46 ms, analysis 0.365 -> 0.23 ms, point values 0.134 ->
+// 0.089 ms; FP32 arithmetic (mixed form): synthesis 0.35 -> 0.30 ms, analysis
+// 0.20 -> 0.16 ms.  The mixed form's h point values stay on the FP32 matrix
+// cores (0.087 ms, as fast as the FP64 FFT).
 hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const* out, int nf) {
   if (c.twy) return launch_ysynth(c, in, out, nf);
   const double* ri[6];
@@ -176,6 +177,7 @@ hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* cons
 
 // Mixed form: FP64 coefficients -> FP32 values on the padded grid, FP32 matrix cores.
 hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const* out, int nf) {
+  if (c.twy) return launch_ysynth32(c, in, out, nf);
   const double* ri[6];
   float* ro[6];
   as_real(in, ri, nf);
@@ -293,6 +295,7 @@ hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out
 
 // Mixed form: FP32 values on the padded grid -> FP64 coefficients, FP32 matrix cores.
 hipError_t launch_analysis32(Ctx& c, const float2* const* in, double2* const* out, int nf) {
+  if (c.twy) return launch_yanalysis32(c, in, out, nf);
   const float* ri[6];
   for (int i = 0; i < nf; ++i) ri[i] = reinterpret_cast<const float*>(in[i]);
   return analysis_t<float, float>(c, c.costabf, ri, out, nf);

@@ -157,9 +157,9 @@ struct Ctx {
   double* wlam[2] = {nullptr, nullptr};
   double2* twx = nullptr;  // exp(-2 pi i m/Lx), m < Lx
   double2* twz = nullptr;  // exp(-2 pi i m/Lz), m < Lz
-  // FFT forms of the FP64 wall-normal transforms (null when 2N has no split
-  // or ncol is not a multiple of 8): the 2N-point twiddles exp(-2 pi i m/2N),
-  // then exp(-2 pi i m/N) (m < N), then exp(i pi k/2N) (k < N)
+  // FFT forms of the wall-normal transforms (null when 2N has no split or
+  // ncol is not a multiple of 16): the 2N-point twiddles exp(-2 pi i m/2N),
+  // then exp(-2 pi i m/N) (m < N), then exp(i pi k/2N) (k < N), FP64
   double2* twy = nullptr;
   // unpadded physical output and plane statistics (allocated on first use)
   double2* phys_c = nullptr;                          // [Ny][Mz][Mx] work
@@ -211,11 +211,13 @@ hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const*
 hipError_t launch_analysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_hassemble(Ctx& c);
 hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const* out, int nf);
-// ydct.hip: the FP64 transforms as FFTs along y (when c.twy is set)
+// ydct.hip: the wall-normal transforms as FFTs along y (when c.twy is set)
 int ydct_length_supported(int N);  // 1 when 2N has an FFT split
 hipError_t launch_ypoints(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_ysynth(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
+hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf);    // FP32 arithmetic
+hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf);  // FP32 compute in the mixed form
 hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);
 hipError_t launch_stream_c2r_unpadded(Ctx& c, const double2* in, double* out);

@@ -1,4 +1,4 @@
-// Wall-normal Chebyshev transforms of the FP64 form through FFTs along y.
+// Wall-normal Chebyshev transforms through FFTs along y.
 //
 // Gauss-Lobatto point values (the synthesis of the h-terms' point values and
 // of the statistics profiles, getall's T0 products PF:1573-1609):
@@ -29,34 +29,55 @@
 // Every transform is real and linear, and the FFT forms above are
 // complex-linear maps that agree with it on real data, so they apply to the
 // complex Fourier coefficients as they are.  O(N log N) work per column
-// instead of the cos-table products' O(N^2) on the FP64 matrix cores.
+// instead of the cos-table products' O(N^2).
 //
-// Layout: a block owns 8 adjacent complex columns (a row of 8 16-byte values
-// is one 128-byte line), and each column's line is transformed by a group of
-// threads with the two-level register FFT of fftreg.h, the twiddles read from
-// the (cached) global tables.
+// The FP64 form runs them in FP64 arithmetic, the mixed form in FP32 (FP64
+// coefficients in and out, FP32 point values).
+//
+// Layout: a block owns TX adjacent complex columns (a row of a block's
+// columns is one 128-byte line of the arithmetic type), and each column's line
+// is transformed by a group of threads with the two-level register FFT of
+// fftreg.h.  The twiddles are generated per thread from two table entries
+// (FP64, also for the FP32 arithmetic): each table read is a vector-memory
+// instruction, and with one read per FFT element the address unit, not the
+// memory, bounded these kernels.
 #include "chf_internal.h"
 #include "fftreg.h"
 
 namespace chf {
 
-constexpr int kYTX = 8;  // columns per block
+template <class V>
+struct YT;
+template <>
+struct YT<double2> {
+  using T = double;
+  static constexpr int TX = 8;
+};
+template <>
+struct YT<float2> {
+  using T = float;
+  static constexpr int TX = 16;
+};
 
-// LDS stride (16-byte values) between the exchange tiles of a block's
-// columns.  The lanes of one LDS pass are adjacent columns (the column index
-// is fastest), so the stride sets the bank pattern: a 16-byte write pass is 8
-// lanes over 8 bank slots, a read pass 16 lanes over 16.  A stride of 9 mod
-// 16 puts a write pass's lanes in distinct slots and leaves reads one
-// collision; the unpadded tile (a multiple of 16 values) put all lanes of a
-// pass in one slot (measured: 0.530 -> 0.487 ms for the padded synthesis at
+// LDS stride (values) between the exchange tiles of a block's columns.  The
+// lanes of one LDS pass are adjacent columns (the column index is fastest), so
+// the stride sets the bank pattern: 8-byte values, a write pass is 16 lanes
+// over 16 8-byte bank slots and a read pass 32 lanes over 32; 16-byte values,
+// 8 and 16 lanes.  A stride of 17 mod 32 (8-byte) or 9 mod 16 (16-byte) values
+// puts a write pass's lanes in distinct slots and leaves reads one collision;
+// the unpadded tile (a multiple of 16 values) put all lanes of a pass in one
+// slot (measured: 0.530 -> 0.487 ms for the FP64 padded synthesis at
 // 256x193x256).
-constexpr int tile_stride(int buf) { return buf + (9 - buf % 16 + 16) % 16; }
+template <class V>
+constexpr int tile_stride(int buf) {
+  return sizeof(V) == 8 ? buf + (17 - buf % 32 + 32) % 32 : buf + (9 - buf % 16 + 16) % 16;
+}
 
 constexpr int ymax(int a, int b) { return a > b ? a : b; }
 
 struct YArgs {
-  const double2* in[6];
-  double2* out[6];
+  const void* in[6];
+  void* out[6];
   long ncol;
   const double2* twL;  // e^{-2 pi i m / 2N}, m < 2N
   const double2* twN;  // e^{-2 pi i m / N},  m < N
@@ -64,25 +85,78 @@ struct YArgs {
   double inv2n;        // 1 / 2N
 };
 
-__device__ __forceinline__ double2 yadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 ysub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 yscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+template <class D, class S>
+__device__ __forceinline__ D ycvt(const S& s) {
+  D d;
+  d.x = (decltype(d.x))s.x;
+  d.y = (decltype(d.y))s.y;
+  return d;
+}
+template <class V>
+__device__ __forceinline__ V yadd(V a, V b) {
+  a.x += b.x;
+  a.y += b.y;
+  return a;
+}
+template <class V>
+__device__ __forceinline__ V ysub(V a, V b) {
+  a.x -= b.x;
+  a.y -= b.y;
+  return a;
+}
+template <class V, class T>
+__device__ __forceinline__ V yscale(V a, T s) {
+  a.x *= s;
+  a.y *= s;
+  return a;
+}
 // Makhoul's pre-twiddle w (a - i b) / 2
-__device__ __forceinline__ double2 ypre(double2 a, double2 b, double2 w) {
-  return yscale(cmulv(make_double2(a.x + b.y, a.y - b.x), w), 0.5);
+template <class V>
+__device__ __forceinline__ V ypre(V a, V b, double2 w) {
+  using T = typename YT<V>::T;
+  V d;
+  d.x = a.x + b.y;
+  d.y = a.y - b.x;
+  return yscale(cmulv(d, ycvt<V>(w)), (T)0.5);
 }
 
-// Gauss-Lobatto point values: the 2N-point FFT (N1 x N2) of the even extension.
+// Step A of fftreg.h's two-level FFT with generated twiddles: thread t needs
+// W^(t k) for k < N2 (W = e^{-2 pi i / (N1 N2)}); from w = W^t and w^4 = W^(4t)
+// (two table reads, 4t < N1 N2), w^k = (w^4)^(k/4) w^(k%4) in FP64, at most
+// seven roundings from the table values.
+template <int N1, int N2, int S, class V>
+__device__ __forceinline__ void ystepA(V (&a)[N2], V* buf, const double2* tw, int t) {
+  static_assert(N2 >= 4, "w^4 = W^(4t) lies in the table");
+  Dft<N2, S, V>::run(a);
+  double2 w1 = tw[t], w4 = tw[4 * t];
+  if (S > 0) {
+    w1.y = -w1.y;
+    w4.y = -w4.y;
+  }
+  const double2 w2 = cmulv(w1, w1), w3 = cmulv(w2, w1);
+  double2 q = make_double2(1.0, 0.0);
+  buf[t * (N2 + 1)] = a[0];
+#pragma unroll
+  for (int k2 = 1; k2 < N2; ++k2) {
+    const int r = k2 & 3;
+    if (r == 0) q = cmulv(q, w4);
+    const double2 w = r == 0 ? q : cmulv(q, r == 1 ? w1 : (r == 2 ? w2 : w3));
+    buf[t * (N2 + 1) + k2] = cmulv(a[k2], ycvt<V>(w));
+  }
+}
+
+// Gauss-Lobatto point values (FP64): the 2N-point FFT (N1 x N2) of the even
+// extension.
 template <int N1, int N2>
-__global__ __launch_bounds__(kYTX* ymax(N1, N2)) void k_ypoints(YArgs a) {
-  constexpr int L = N1 * N2, N = L / 2, SB = tile_stride(N1 * (N2 + 1)), TX = kYTX;
+__global__ __launch_bounds__(YT<double2>::TX* ymax(N1, N2)) void k_ypoints(YArgs a) {
+  constexpr int L = N1 * N2, N = L / 2, TX = YT<double2>::TX, SB = tile_stride<double2>(N1 * (N2 + 1));
   extern __shared__ __align__(16) unsigned char smraw[];
   const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
   double2* buf = reinterpret_cast<double2*>(smraw) + tx * SB;
   const long col = xcd_order(blockIdx.x, gridDim.x) * TX + tx;
   const long nc = a.ncol;
-  const double2* in = a.in[blockIdx.z];
-  double2* out = a.out[blockIdx.z];
+  const double2* in = static_cast<const double2*>(a.in[blockIdx.z]);
+  double2* out = static_cast<double2*>(a.out[blockIdx.z]);
   if (t < N1) {
     double2 x[N2];
 #pragma unroll
@@ -90,7 +164,7 @@ __global__ __launch_bounds__(kYTX* ymax(N1, N2)) void k_ypoints(YArgs a) {
       const int m = n2 * N1 + t;
       x[n2] = in[(long)(m <= N ? m : L - m) * nc + col];
     }
-    fft_stepA<N1, N2, -1>(x, buf, a.twL, t);
+    ystepA<N1, N2, -1>(x, buf, a.twL, t);
   }
   __syncthreads();
   if (t < N2) {
@@ -106,74 +180,108 @@ __global__ __launch_bounds__(kYTX* ymax(N1, N2)) void k_ypoints(YArgs a) {
 }
 
 // Padded synthesis (ANA = false) or analysis (ANA = true): the 2N-point FFT
-// (A1 x A2), then the N-point FFT (A1 x A2/2) through the same LDS tile.  Each
-// FFT reads its inputs when it starts (a variant that formed the second FFT's
-// inputs from the first one's loads, so that a block waits for memory once,
-// needed 222-253 VGPRs instead of 156-172 and ran slower).
-template <int A1, int A2, bool ANA>
-__global__ __launch_bounds__(kYTX* ymax(A1, A2)) void k_ycheb(YArgs a) {
-  constexpr int L = A1 * A2, N = L / 2, B2 = A2 / 2, TX = kYTX;
+// (A1 x A2), then the N-point FFT (A1 x A2/2) through the same LDS tile.
+// With these splits thread t of a column holds, for the first FFT, the
+// extended sequence at m = n A1 + t, which is index k = m (n < A2/2) and
+// index N - k (at n + A2/2) of the second.  SINGLE: the second FFT's inputs
+// are formed from the first one's loads (each input read once, more
+// registers); otherwise they are read again when the second FFT starts.
+template <int A1, int A2, class V, class TI, class TO, bool ANA, bool SINGLE>
+__global__ __launch_bounds__(YT<V>::TX* ymax(A1, A2)) void k_ycheb(YArgs a) {
+  using T = typename YT<V>::T;
+  constexpr int L = A1 * A2, N = L / 2, B2 = A2 / 2, TX = YT<V>::TX;
   static_assert(A2 % 2 == 0, "the second FFT is A1 x A2/2");
-  constexpr int SB = tile_stride(A1 * (A2 + 1));
+  constexpr int SB = tile_stride<V>(A1 * (A2 + 1));
   extern __shared__ __align__(16) unsigned char smraw[];
   const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
-  double2* buf = reinterpret_cast<double2*>(smraw) + tx * SB;
+  V* buf = reinterpret_cast<V*>(smraw) + tx * SB;
   const long col = xcd_order(blockIdx.x, gridDim.x) * TX + tx;
   const long nc = a.ncol;
-  const double2* in = a.in[blockIdx.z];
-  double2* out = a.out[blockIdx.z];
-  auto ld = [&](int r) { return in[(long)r * nc + col]; };
-  auto st = [&](int r, double2 v) { out[(long)r * nc + col] = v; };
-  auto hodd = [&](int q) { return q == 0 ? ysub(ld(0), ld(2 * N)) : yscale(ysub(ld(q), ld(2 * N - q)), 2.0); };
+  const TI* in = static_cast<const TI*>(a.in[blockIdx.z]);
+  TO* out = static_cast<TO*>(a.out[blockIdx.z]);
+  auto ld = [&](int r) { return ycvt<V>(in[(long)r * nc + col]); };
+  auto st = [&](int r, V v) { out[(long)r * nc + col] = ycvt<TO>(v); };
+  // H_q of the analysis from its two points
+  auto hodd = [&](int q, V f, V g) { return q == 0 ? ysub(f, g) : yscale(ysub(f, g), (T)2); };
 
-  // 2N-point FFT of the even extension: even points / even coefficients
+  V y[B2];  // the N-point FFT's inputs (threads t < A1)
+  // Makhoul pre-twiddles w_k, k = n A1 + t: w_t (w_A1)^n, generated in FP64
+  const double2 wstep = a.wq[A1];
   if (t < A1) {
-    double2 x[A2];
+    double2 wk = a.wq[t];
+    V x[A2];
+    if constexpr (SINGLE) {
 #pragma unroll
-    for (int n2 = 0; n2 < A2; ++n2) {
-      const int m = n2 * A1 + t;
-      const int p = m <= N ? m : L - m;
-      x[n2] = ANA ? yadd(ld(p), ld(2 * N - p)) : ld(p);  // analysis, p = N: G_N = 2 f_N
-    }
-    fft_stepA<A1, A2, -1>(x, buf, a.twL, t);
-  }
-  __syncthreads();
-  if (t < A2) {
-    double2 b[A1];
-    fft_stepB<A1, A2, -1>(b, buf, t);
-    if constexpr (!ANA) {
-      const double2 a0 = ld(0), aN = ld(N);
-#pragma unroll
-      for (int k1 = 0; k1 < A1; ++k1) {
-        const int j = k1 * A2 + t;
-        if (j <= N) st(2 * j, yscale(yadd(yadd(b[k1], a0), (j & 1) ? yscale(aN, -1.0) : aN), 0.5));
+      for (int n = 0; n < B2; ++n) {
+        const int k = n * A1 + t;          // < N: m = k, and m = N + k at n + B2
+        const int q = k == 0 ? N : N - k;  // the even extension at m = N + k
+        if constexpr (!ANA) {
+          x[n] = ld(k);
+          x[n + B2] = ld(q);
+          y[n] = (k == 0) ? x[n] : ypre(x[n], x[n + B2], wk);
+        } else {
+          const V f0 = ld(k), g0 = ld(2 * N - k), f1 = ld(q), g1 = ld(2 * N - q);
+          x[n] = yadd(f0, g0);
+          x[n + B2] = yadd(f1, g1);  // q = N: 2 f_N
+          const V h0 = hodd(k, f0, g0);
+          y[n] = (k == 0) ? h0 : ypre(h0, hodd(q, f1, g1), wk);
+        }
+        wk = cmulv(wk, wstep);
       }
     } else {
 #pragma unroll
+      for (int n2 = 0; n2 < A2; ++n2) {
+        const int m = n2 * A1 + t;
+        const int p = m <= N ? m : L - m;
+        x[n2] = ANA ? yadd(ld(p), ld(2 * N - p)) : ld(p);  // analysis, p = N: G_N = 2 f_N
+      }
+    }
+    ystepA<A1, A2, -1>(x, buf, a.twL, t);
+  }
+  __syncthreads();
+  // 2N-point FFT of the even extension: even points / even coefficients
+  if (t < A2) {
+    V b[A1];
+    fft_stepB<A1, A2, -1>(b, buf, t);
+    if constexpr (!ANA) {
+      const V a0 = ld(0), aN = ld(N);
+#pragma unroll
+      for (int k1 = 0; k1 < A1; ++k1) {
+        const int j = k1 * A2 + t;
+        if (j <= N) st(2 * j, yscale(yadd(yadd(b[k1], a0), (j & 1) ? yscale(aN, (T)-1) : aN), (T)0.5));
+      }
+    } else {
+      const T s0 = (T)(0.5 * a.inv2n), s1 = (T)a.inv2n;
+#pragma unroll
       for (int k1 = 0; k1 < A1; ++k1) {
         const int m = k1 * A2 + t;
-        if (2 * m <= N) st(2 * m, yscale(b[k1], m == 0 ? 0.5 * a.inv2n : a.inv2n));
+        if (2 * m <= N) st(2 * m, yscale(b[k1], m == 0 ? s0 : s1));
       }
     }
   }
   __syncthreads();  // the tile is reused
   // N-point FFT of the type-III DCT: odd points / odd coefficients
   if (t < A1) {
-    double2 y[B2];
+    if constexpr (!SINGLE) {
+      // the inputs read again (the first FFT's are no longer live)
+      double2 wk = a.wq[t];
 #pragma unroll
-    for (int n2 = 0; n2 < B2; ++n2) {
-      const int k = n2 * A1 + t;
-      if constexpr (!ANA) {
-        y[n2] = (k == 0) ? ld(0) : ypre(ld(k), ld(N - k), a.wq[k]);
-      } else {
-        y[n2] = (k == 0) ? hodd(0) : ypre(hodd(k), hodd(N - k), a.wq[k]);
+      for (int n = 0; n < B2; ++n) {
+        const int k = n * A1 + t;
+        if constexpr (!ANA) {
+          y[n] = (k == 0) ? ld(0) : ypre(ld(k), ld(N - k), wk);
+        } else {
+          const V h0 = hodd(k, ld(k), ld(2 * N - k));
+          y[n] = (k == 0) ? h0 : ypre(h0, hodd(N - k, ld(N - k), ld(N + k)), wk);
+        }
+        wk = cmulv(wk, wstep);
       }
     }
-    fft_stepA<A1, B2, +1>(y, buf, a.twN, t);
+    ystepA<A1, B2, +1>(y, buf, a.twN, t);
   }
   __syncthreads();
   if (t < B2) {
-    double2 v[A1];
+    V v[A1];
     fft_stepB<A1, B2, +1>(v, buf, t);
 #pragma unroll
     for (int k1 = 0; k1 < A1; ++k1) {
@@ -182,31 +290,42 @@ __global__ __launch_bounds__(kYTX* ymax(A1, A2)) void k_ycheb(YArgs a) {
       if constexpr (!ANA) {
         st(2 * l + 1, v[k1]);
       } else {
-        if (2 * l + 1 <= N) st(2 * l + 1, yscale(v[k1], a.inv2n));
+        if (2 * l + 1 <= N) st(2 * l + 1, yscale(v[k1], (T)a.inv2n));
       }
     }
   }
 }
 
 template <class K>
-static hipError_t launch_y(Ctx& c, K kernel, int nthreads, int tile, const YArgs& a, int nf) {
-  if (c.d.ncol % kYTX) return hipErrorNotSupported;
-  const size_t shm = sizeof(double2) * (size_t)kYTX * tile_stride(tile);
+static hipError_t launch_y(Ctx& c, K kernel, int tx, int nthreads, size_t shm, const YArgs& a, int nf) {
+  if (c.d.ncol % tx) return hipErrorNotSupported;
   if (shm > 65536) {
     const hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kernel, dim3((unsigned)(c.d.ncol / kYTX), 1, nf), dim3(nthreads), shm, c.stream, a);
+  hipLaunchKernelGGL(kernel, dim3((unsigned)(c.d.ncol / tx), 1, nf), dim3(nthreads), shm, c.stream, a);
   return hipGetLastError();
 }
 
+// Which form of the second FFT's inputs each kernel takes (measured at
+// 256x193x256, ms, single load / read again: FP32 synthesis 0.296 / 0.336,
+// analysis 0.156 / 0.195; FP64 synthesis 0.540 / 0.458, analysis 0.230 /
+// 0.286).
+template <class V, bool ANA>
+constexpr bool single_load() {
+  return sizeof(V) == 8 || ANA;
+}
+
 // mode 0: point values, 1: padded synthesis, 2: analysis; A1 x A2 = 2N
-template <int A1, int A2>
+template <int A1, int A2, class V, class TI, class TO>
 static hipError_t run_y(Ctx& c, int mode, const YArgs& a, int nf) {
-  constexpr int nt = kYTX * ymax(A1, A2), tile = A1 * (A2 + 1);
-  if (mode == 0) return launch_y(c, k_ypoints<A1, A2>, nt, tile, a, nf);
-  if (mode == 1) return launch_y(c, k_ycheb<A1, A2, false>, nt, tile, a, nf);
-  return launch_y(c, k_ycheb<A1, A2, true>, nt, tile, a, nf);
+  constexpr int TX = YT<V>::TX, nt = TX * ymax(A1, A2);
+  const size_t shm = sizeof(V) * (size_t)TX * tile_stride<V>(A1 * (A2 + 1));
+  if constexpr (sizeof(V) == 16) {
+    if (mode == 0) return launch_y(c, k_ypoints<A1, A2>, TX, nt, shm, a, nf);
+  }
+  if (mode == 1) return launch_y(c, k_ycheb<A1, A2, V, TI, TO, false, single_load<V, false>()>, TX, nt, shm, a, nf);
+  return launch_y(c, k_ycheb<A1, A2, V, TI, TO, true, single_load<V, true>()>, TX, nt, shm, a, nf);
 }
 
 // 1 when 2N has a split below (and so an FFT form of the transforms).
@@ -217,7 +336,8 @@ int ydct_length_supported(int N) {
   }
 }
 
-static hipError_t run_ydct(Ctx& c, int mode, const double2* const* in, double2* const* out, int nf) {
+template <class V, class TI, class TO>
+static hipError_t run_ydct(Ctx& c, int mode, const TI* const* in, TO* const* out, int nf) {
   YArgs a{};
   for (int f = 0; f < nf; ++f) {
     a.in[f] = in[f];
@@ -230,25 +350,31 @@ static hipError_t run_ydct(Ctx& c, int mode, const double2* const* in, double2* 
   a.wq = c.twy + 3 * N;
   a.inv2n = 1.0 / (2.0 * N);
   switch (2 * N) {
-    case 64: return run_y<8, 8>(c, mode, a, nf);
-    case 96: return run_y<8, 12>(c, mode, a, nf);
-    case 128: return run_y<8, 16>(c, mode, a, nf);
-    case 192: return run_y<12, 16>(c, mode, a, nf);
-    case 256: return run_y<16, 16>(c, mode, a, nf);
-    case 384: return run_y<16, 24>(c, mode, a, nf);
-    case 512: return run_y<16, 32>(c, mode, a, nf);
+    case 64: return run_y<8, 8, V, TI, TO>(c, mode, a, nf);
+    case 96: return run_y<8, 12, V, TI, TO>(c, mode, a, nf);
+    case 128: return run_y<8, 16, V, TI, TO>(c, mode, a, nf);
+    case 192: return run_y<12, 16, V, TI, TO>(c, mode, a, nf);
+    case 256: return run_y<16, 16, V, TI, TO>(c, mode, a, nf);
+    case 384: return run_y<16, 24, V, TI, TO>(c, mode, a, nf);
+    case 512: return run_y<16, 32, V, TI, TO>(c, mode, a, nf);
     default: return hipErrorNotSupported;
   }
 }
 
 hipError_t launch_ypoints(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return run_ydct(c, 0, in, out, nf);
+  return run_ydct<double2>(c, 0, in, out, nf);
 }
 hipError_t launch_ysynth(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return run_ydct(c, 1, in, out, nf);
+  return run_ydct<double2>(c, 1, in, out, nf);
 }
 hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return run_ydct(c, 2, in, out, nf);
+  return run_ydct<double2>(c, 2, in, out, nf);
+}
+hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf) {
+  return run_ydct<float2>(c, 1, in, out, nf);
+}
+hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf) {
+  return run_ydct<float2>(c, 2, in, out, nf);
 }
 
 }  // namespace chf
