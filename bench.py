@@ -56,8 +56,8 @@ def kernel_work(d, precision="fp64", yfft=False):
     FP64 in the FP64 form and FP32 (8 B) in the mixed form.  Flops count the
     useful multiply-adds (2 flops each) of the parity-split contractions.
     With yfft (the wall-normal transforms run as FFTs along y) the padded
-    synthesis and the analysis, and the FP64 form's point values, have
-    O(N log N) work per column and are priced by their bytes alone.
+    synthesis, the analysis and the point values have O(N log N) work per
+    column and are priced by their bytes alone.
     """
     w = 16 if precision == "fp64" else 8
     S = d["Ny"] * d["ncol"] * 16            # spectral field (FP64)
@@ -70,7 +70,6 @@ def kernel_work(d, precision="fp64", yfft=False):
     inv_v_bytes = (d["nclass"] - 1) * (ne * ne + no * no) * 8   # one per k^2 class; k^2 = 0 has no v system
     ws_flops = 2.0 * col * ((ne - 1) * (ne - 1) + ne * (ne - 1) + (no - 1) * (no - 1) + no * (no - 1))
     yf = 0.0 if yfft else 1.0
-    yp = 0.0 if yfft and precision == "fp64" else 1.0
     return {
         "synth_padded": (6 * (S + Y), yf * 2.0 * Ph * Ny * col * 6),
         "zfft_inv": (6 * (Y + Z), 0.0),
@@ -78,7 +77,7 @@ def kernel_work(d, precision="fp64", yfft=False):
         "zfft_fwd": (3 * (Z + Y), 0.0),
         "analysis": (3 * (Y + S), yf * 2.0 * Ny * Ph * col * 3),
         "hassemble": (5 * S + 4 * Ny * 16, 0.0),   # plus the mean-mode point block
-        "synth_points": (4 * S, yp * 2.0 * Pn * Ny * col * 2),
+        "synth_points": (4 * S, yf * 2.0 * Pn * Ny * col * 2),
         "rhs_v": (S + 2 * (Ny - 4) / Ny * S + rv_rows / Ny * S,
                   2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
         "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S,

@@ -286,8 +286,8 @@ int autotune(Ctx* c) {
   };
   std::vector<Stage> st;
   if (c->twy) {
-    // the padded synthesis and the analysis (and the FP64 form's point
-    // values) run as FFTs, which have no tile choice
+    // the padded synthesis, the analysis and the point values run as FFTs,
+    // which have no tile choice
   } else if (c->prm.precision == 0) {
     st.push_back({K_SYNTH_PADDED, [c] {
                     const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
@@ -307,7 +307,7 @@ int autotune(Ctx* c) {
                     return launch_analysis32(*c, y3, c->Lc, 3);
                   }});
   }
-  if (!(c->twy && c->prm.precision == 0))
+  if (!c->twy)
     st.push_back({K_SYNTH_POINTS, [c] {
                     const double2* hc[2] = {c->hvc, c->hgc};
                     double2* ho[2] = {c->hv[0], c->hg[0]};

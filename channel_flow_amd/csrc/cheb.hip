@@ -140,8 +140,7 @@ static void as_real(const double2* const* in, const double** out, int nf) {
 // products on the matrix cores otherwise.  At 256x193x256, FP64: padded
 // synthesis 0.625 -> 0.46 ms, analysis 0.365 -> 0.23 ms, point values 0.134 ->
 // 0.089 ms; FP32 arithmetic (mixed form): synthesis 0.35 -> 0.30 ms, analysis
-// 0.20 -> 0.16 ms.  The mixed form's h point values stay on the FP32 matrix
-// cores (0.087 ms, as fast as the FP64 FFT).
+// 0.20 -> 0.16 ms, h point values 0.092 -> 0.082 ms.
 hipError_t launch_synth_padded(Ctx& c, const double2* const* in, double2* const* out, int nf) {
   if (c.twy) return launch_ysynth(c, in, out, nf);
   const double* ri[6];
@@ -161,13 +160,15 @@ hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const*
 }
 
 // h_v, h_g at the N+1 points (the AB2 forcing history): FP64 arrays in both
-// forms, computed on the FP32 matrix cores in the mixed form.  The values
+// forms, computed in FP32 arithmetic in the mixed form (FFTs, or the FP32
+// matrix cores for degrees without an FFT split).  The values
 // enter the solve only through dt (3 h^n - h^{n-1}) / 2, as the FP32 Lamb
 // vector they are formed from does (measured at 256x193x256 over 100 steps,
 // against the FP64 form: field errors within 3% of those with FP64 point
 // values, wall shear 3.1e-10 / 1.2e-9 against 3.6e-10 / 1.0e-9).
 hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf) {
   if (c.prm.precision == 0) return launch_synth_points(c, in, out, nf);
+  if (c.twy) return launch_ypoints32(c, in, out, nf);
   const double* ri[6];
   double* ro[6];
   as_real(in, ri, nf);

@@ -216,7 +216,8 @@ int ydct_length_supported(int N);  // 1 when 2N has an FFT split
 hipError_t launch_ypoints(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_ysynth(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
-hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf);    // FP32 arithmetic
+hipError_t launch_ypoints32(Ctx& c, const double2* const* in, double2* const* out, int nf);   // FP32 arithmetic
+hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf);
 hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf);  // FP32 compute in the mixed form
 hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);

@@ -145,36 +145,38 @@ __device__ __forceinline__ void ystepA(V (&a)[N2], V* buf, const double2* tw, in
   }
 }
 
-// Gauss-Lobatto point values (FP64): the 2N-point FFT (N1 x N2) of the even
-// extension.
-template <int N1, int N2>
-__global__ __launch_bounds__(YT<double2>::TX* ymax(N1, N2)) void k_ypoints(YArgs a) {
-  constexpr int L = N1 * N2, N = L / 2, TX = YT<double2>::TX, SB = tile_stride<double2>(N1 * (N2 + 1));
+// Gauss-Lobatto point values, FP64 coefficients to FP64 values in arithmetic
+// V: the 2N-point FFT (N1 x N2) of the even extension.
+template <int N1, int N2, class V>
+__global__ __launch_bounds__(YT<V>::TX* ymax(N1, N2)) void k_ypoints(YArgs a) {
+  using T = typename YT<V>::T;
+  constexpr int L = N1 * N2, N = L / 2, TX = YT<V>::TX, SB = tile_stride<V>(N1 * (N2 + 1));
   extern __shared__ __align__(16) unsigned char smraw[];
   const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
-  double2* buf = reinterpret_cast<double2*>(smraw) + tx * SB;
+  V* buf = reinterpret_cast<V*>(smraw) + tx * SB;
   const long col = xcd_order(blockIdx.x, gridDim.x) * TX + tx;
   const long nc = a.ncol;
   const double2* in = static_cast<const double2*>(a.in[blockIdx.z]);
   double2* out = static_cast<double2*>(a.out[blockIdx.z]);
   if (t < N1) {
-    double2 x[N2];
+    V x[N2];
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) {
       const int m = n2 * N1 + t;
-      x[n2] = in[(long)(m <= N ? m : L - m) * nc + col];
+      x[n2] = ycvt<V>(in[(long)(m <= N ? m : L - m) * nc + col]);
     }
     ystepA<N1, N2, -1>(x, buf, a.twL, t);
   }
   __syncthreads();
   if (t < N2) {
-    double2 b[N1];
+    V b[N1];
     fft_stepB<N1, N2, -1>(b, buf, t);
-    const double2 a0 = in[col], aN = in[(long)N * nc + col];
+    const V a0 = ycvt<V>(in[col]), aN = ycvt<V>(in[(long)N * nc + col]);
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
       const int j = k1 * N2 + t;
-      if (j <= N) out[(long)j * nc + col] = yscale(yadd(yadd(b[k1], a0), (j & 1) ? yscale(aN, -1.0) : aN), 0.5);
+      if (j <= N)
+        out[(long)j * nc + col] = ycvt<double2>(yscale(yadd(yadd(b[k1], a0), (j & 1) ? yscale(aN, (T)-1) : aN), (T)0.5));
     }
   }
 }
@@ -321,9 +323,7 @@ template <int A1, int A2, class V, class TI, class TO>
 static hipError_t run_y(Ctx& c, int mode, const YArgs& a, int nf) {
   constexpr int TX = YT<V>::TX, nt = TX * ymax(A1, A2);
   const size_t shm = sizeof(V) * (size_t)TX * tile_stride<V>(A1 * (A2 + 1));
-  if constexpr (sizeof(V) == 16) {
-    if (mode == 0) return launch_y(c, k_ypoints<A1, A2>, TX, nt, shm, a, nf);
-  }
+  if (mode == 0) return launch_y(c, k_ypoints<A1, A2, V>, TX, nt, shm, a, nf);
   if (mode == 1) return launch_y(c, k_ycheb<A1, A2, V, TI, TO, false, single_load<V, false>()>, TX, nt, shm, a, nf);
   return launch_y(c, k_ycheb<A1, A2, V, TI, TO, true, single_load<V, true>()>, TX, nt, shm, a, nf);
 }
@@ -369,6 +369,9 @@ hipError_t launch_ysynth(Ctx& c, const double2* const* in, double2* const* out, 
 }
 hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
   return run_ydct<double2>(c, 2, in, out, nf);
+}
+hipError_t launch_ypoints32(Ctx& c, const double2* const* in, double2* const* out, int nf) {
+  return run_ydct<float2>(c, 0, in, out, nf);
 }
 hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf) {
   return run_ydct<float2>(c, 1, in, out, nf);

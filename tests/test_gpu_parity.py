@@ -278,15 +278,14 @@ def test_autotuner_reports_choices(require_gpu):
 
 
 def test_fft_forms_have_no_tile_choice(require_gpu):
-    """At N = 192 both forms run the padded synthesis and the analysis as FFTs
-    (no tile choice), the FP64 form its point values too; the mixed form's h
-    point values stay on the FP32 matrix cores."""
+    """At N = 192 both forms run the padded synthesis, the analysis and the
+    point values as FFTs, which have no tile choice."""
     cfg = dict(nnx=32, nny=32, nnz=192, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
                dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=21)
     with make_channel(cfg, "fp64") as ch:
         assert set(ch.tile_choices()) == {"rhs_v", "rhs_o"}, ch.tile_choices()
     with make_channel(cfg, "mixed") as ch:
-        assert set(ch.tile_choices()) == {"synth_points", "rhs_v", "rhs_o"}, ch.tile_choices()
+        assert set(ch.tile_choices()) == {"rhs_v", "rhs_o"}, ch.tile_choices()
 
 
 def test_fft_and_matrix_transforms_agree(require_gpu, monkeypatch):
