@@ -286,8 +286,8 @@ int autotune(Ctx* c) {
   };
   std::vector<Stage> st;
   if (c->twy) {
-    // the padded synthesis, the analysis and the point values run as FFTs,
-    // which have no tile choice
+    // the padded synthesis, the analysis, the point values and the
+    // right-hand sides run as FFTs, which have no tile choice
   } else if (c->prm.precision == 0) {
     st.push_back({K_SYNTH_PADDED, [c] {
                     const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
@@ -313,8 +313,10 @@ int autotune(Ctx* c) {
                     double2* ho[2] = {c->hv[0], c->hg[0]};
                     return launch_synth_points_h(*c, hc, ho, 2);
                   }});
-  st.push_back({K_RHS_V, [c] { return launch_rhs(*c, 0); }});
-  st.push_back({K_RHS_O, [c] { return launch_rhs(*c, 1); }});
+  if (!c->twy) {
+    st.push_back({K_RHS_V, [c] { return launch_rhs(*c, 0); }});
+    st.push_back({K_RHS_O, [c] { return launch_rhs(*c, 1); }});
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   CHF_TRY(hipEventCreate(&e0));
   CHF_TRY(hipEventCreate(&e1));

@@ -401,6 +401,11 @@ static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const dou
 
 hipError_t launch_rhs(Ctx& c, int which) {
   const int nxt = c.hcur, prv = 1 - c.hcur;  // h^n, h^{n-1}
+  if (c.twy) {
+    // FFT form (ydct.hip): the bracket's point values by the type-I DCT
+    if (which == 0) return launch_yrhs(c, 0, c.gv, c.hv[nxt], c.hv[prv], nullptr, nullptr, c.rvT);
+    return launch_yrhs(c, 1, c.go, c.hg[nxt], c.hg[prv], c.roE, c.roO, nullptr);
+  }
   if (which == 0) return rhs_syn<2>(c, c.gv, c.hv[nxt], c.hv[prv], nullptr, nullptr, c.rvT);
   return rhs_syn<1>(c, c.go, c.hg[nxt], c.hg[prv], c.roE, c.roO, nullptr);
 }

@@ -216,6 +216,8 @@ int ydct_length_supported(int N);  // 1 when 2N has an FFT split
 hipError_t launch_ypoints(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_ysynth(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
+hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hn, const double2* hp, double2* oE,
+                       double2* oO, double2* oT);  // which: 0 = v, 1 = omega_y
 hipError_t launch_ypoints32(Ctx& c, const double2* const* in, double2* const* out, int nf);   // FP32 arithmetic
 hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf);
 hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);

@@ -298,8 +298,88 @@ __global__ __launch_bounds__(YT<V>::TX* ymax(A1, A2)) void k_ycheb(YArgs a) {
   }
 }
 
-template <class K>
-static hipError_t launch_y(Ctx& c, K kernel, int tx, int nthreads, size_t shm, const YArgs& a, int nf) {
+// Right-hand sides of the v (POFF = 2) and omega_y (POFF = 1) half systems
+// (PF:252-257, 335-340; the explicit operator as the bracket g formed by the
+// kinematics pass): f = sum_k g_k T_k at the Gauss-Lobatto points (the type-I
+// DCT, FP64), plus the AB2 forcing F = 3 dt/2 h^n - dt/2 h^(n-1), folded by
+// parity at the point pairs p = r + POFF, q = N - p: even row r
+// ((f+F)_p + (f+F)_q)/2 (rows r < ne - POFF), odd row r ((f+F)_p - (f+F)_q)/2
+// (r < no - POFF).  The point values pass through the LDS tile so that each
+// thread sees both points of its pairs.  v: each mode's rows as one record,
+// [column][Rv: even rows, then odd rows]; omega_y: [row][column] per parity.
+struct YRhsArgs {
+  const double2* g;
+  const double2 *hn, *hp;
+  double2 *outT, *outE, *outO;
+  long ncol;
+  int Rv, ne, no;
+  double dt;
+  const double2* twL;
+};
+
+template <int N1, int N2, int POFF>
+__global__ __launch_bounds__(YT<double2>::TX* ymax(N1, N2)) void k_yrhs(YRhsArgs a) {
+  constexpr int L = N1 * N2, N = L / 2, TX = YT<double2>::TX, G = ymax(N1, N2);
+  constexpr int SB = tile_stride<double2>(N1 * (N2 + 1));
+  static_assert(SB > N, "the point values fit the exchange tile");
+  extern __shared__ __align__(16) unsigned char smraw[];
+  const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
+  double2* buf = reinterpret_cast<double2*>(smraw) + tx * SB;
+  const long col = xcd_order(blockIdx.x, gridDim.x) * TX + tx;
+  const long nc = a.ncol;
+  if (t < N1) {
+    double2 x[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      const int m = n2 * N1 + t;
+      x[n2] = a.g[(long)(m <= N ? m : L - m) * nc + col];
+    }
+    ystepA<N1, N2, -1>(x, buf, a.twL, t);
+  }
+  __syncthreads();
+  double2 b[N1];
+  if (t < N2) fft_stepB<N1, N2, -1>(b, buf, t);
+  __syncthreads();  // the tile now takes the point values
+  if (t < N2) {
+    const double2 g0 = a.g[col], gN = a.g[(long)N * nc + col];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int j = k1 * N2 + t;
+      if (j <= N) buf[j] = yscale(yadd(yadd(b[k1], g0), (j & 1) ? yscale(gN, -1.0) : gN), 0.5);
+    }
+  }
+  __syncthreads();
+  const double c1 = 1.5 * a.dt, c0 = 0.5 * a.dt;
+  auto fF = [&](int j) {
+    const double2 hn = a.hn[(long)j * nc + col], hp = a.hp[(long)j * nc + col];
+    const double2 f = buf[j];
+    return make_double2(f.x + (c1 * hn.x - c0 * hp.x), f.y + (c1 * hn.y - c0 * hp.y));
+  };
+  const int Me = a.ne - POFF, Mo = a.no - POFF;
+  for (int r = t; r < Me; r += G) {
+    const int p = r + POFF, q = N - p;
+    const double2 vp = fF(p);
+    double2 ev, od = make_double2(0.0, 0.0);
+    if (q == p) {
+      ev = vp;
+    } else {
+      const double2 vq = fF(q);
+      ev = yscale(yadd(vp, vq), 0.5);
+      od = yscale(ysub(vp, vq), 0.5);
+    }
+    if constexpr (POFF == 2) {
+      double2* rec = a.outT + col * a.Rv;
+      rec[r] = ev;
+      if (r < Mo) rec[Me + r] = od;
+    } else {
+      a.outE[(long)r * nc + col] = ev;
+      if (r < Mo) a.outO[(long)r * nc + col] = od;
+    }
+  }
+}
+
+template <class K, class A>
+static hipError_t launch_y(Ctx& c, K kernel, int tx, int nthreads, size_t shm, const A& a, int nf) {
   if (c.d.ncol % tx) return hipErrorNotSupported;
   if (shm > 65536) {
     const hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -370,6 +450,41 @@ hipError_t launch_ysynth(Ctx& c, const double2* const* in, double2* const* out, 
 hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* out, int nf) {
   return run_ydct<double2>(c, 2, in, out, nf);
 }
+template <int N1, int N2>
+static hipError_t run_yrhs(Ctx& c, int which, const YRhsArgs& a) {
+  constexpr int TX = YT<double2>::TX, nt = TX * ymax(N1, N2);
+  const size_t shm = sizeof(double2) * (size_t)TX * tile_stride<double2>(N1 * (N2 + 1));
+  if (which == 0) return launch_y(c, k_yrhs<N1, N2, 2>, TX, nt, shm, a, 1);
+  return launch_y(c, k_yrhs<N1, N2, 1>, TX, nt, shm, a, 1);
+}
+
+hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hn, const double2* hp, double2* oE,
+                       double2* oO, double2* oT) {
+  YRhsArgs a{};
+  a.g = g;
+  a.hn = hn;
+  a.hp = hp;
+  a.outT = oT;
+  a.outE = oE;
+  a.outO = oO;
+  a.ncol = c.d.ncol;
+  a.Rv = c.d.ne + c.d.no - 4;
+  a.ne = c.d.ne;
+  a.no = c.d.no;
+  a.dt = c.prm.dt;
+  a.twL = c.twy;
+  switch (2 * c.d.N) {
+    case 64: return run_yrhs<8, 8>(c, which, a);
+    case 96: return run_yrhs<8, 12>(c, which, a);
+    case 128: return run_yrhs<8, 16>(c, which, a);
+    case 192: return run_yrhs<12, 16>(c, which, a);
+    case 256: return run_yrhs<16, 16>(c, which, a);
+    case 384: return run_yrhs<16, 24>(c, which, a);
+    case 512: return run_yrhs<16, 32>(c, which, a);
+    default: return hipErrorNotSupported;
+  }
+}
+
 hipError_t launch_ypoints32(Ctx& c, const double2* const* in, double2* const* out, int nf) {
   return run_ydct<float2>(c, 0, in, out, nf);
 }

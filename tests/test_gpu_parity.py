@@ -278,14 +278,15 @@ def test_autotuner_reports_choices(require_gpu):
 
 
 def test_fft_forms_have_no_tile_choice(require_gpu):
-    """At N = 192 both forms run the padded synthesis, the analysis and the
-    point values as FFTs, which have no tile choice."""
+    """At N = 192 both forms run every wall-normal transform (padded
+    synthesis, analysis, point values, right-hand sides) as FFTs, which have
+    no tile choice."""
     cfg = dict(nnx=32, nny=32, nnz=192, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
                dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=21)
     with make_channel(cfg, "fp64") as ch:
-        assert set(ch.tile_choices()) == {"rhs_v", "rhs_o"}, ch.tile_choices()
+        assert ch.tile_choices() == {}, ch.tile_choices()
     with make_channel(cfg, "mixed") as ch:
-        assert set(ch.tile_choices()) == {"rhs_v", "rhs_o"}, ch.tile_choices()
+        assert ch.tile_choices() == {}, ch.tile_choices()
 
 
 def test_fft_and_matrix_transforms_agree(require_gpu, monkeypatch):
@@ -339,17 +340,20 @@ def test_fft_and_matrix_transforms_agree_mixed(require_gpu, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_tile_choices_bit_identical(require_gpu, precision):
+def test_tile_choices_bit_identical(require_gpu, precision, monkeypatch):
     """Every column-block choice of the tuned contractions (forced through
     chf_set_tile_choice) gives bit-identical states after initial() and two
     steps, at N = 192 (row blocks that admit 1, 2 and 4 column blocks, the
-    centre rows summed outside the matrix cores)."""
+    centre rows summed outside the matrix cores), with the wall-normal
+    transforms as cos-table products (CHF_Y_TRANSFORMS=matrix)."""
+    monkeypatch.setenv("CHF_Y_TRANSFORMS", "matrix")
     cfg = dict(nnx=32, nny=32, nnz=192, aphi=1.0, beta=0.5, re=7940.0, re_p=2.0 / 7940.0,
                dt=2.5e-4, u_c=1.0, rms=0.1, kmax=6.0, slope=-4.0 / 3.0, seed=24)
     v, oy, au0, w0 = ic_of(cfg)
     states = {}
     for cb in (1, 2, 4):
         with make_channel(cfg, precision) as ch:
+            assert len(ch.tile_choices()) == 5, ch.tile_choices()
             for name in ch.tile_choices():
                 ch.set_tile_choice(name, cb)
             ch.initial(v, oy, au0, w0)
