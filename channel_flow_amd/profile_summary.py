@@ -27,6 +27,8 @@ import sys
 def short_name(name: str) -> str:
     name = re.sub(r"\(.*\)$", "", name)
     name = name.replace("chf::", "")
+    # HIP_vector_type<double, 2u> -> double2, so template arguments fit
+    name = re.sub(r"HIP_vector_type<(\w+), (\d)u>", r"\g<1>\g<2>", name)
     return name[:90]
 
 
