@@ -62,7 +62,8 @@ typedef struct chf_state {
  * direct solves to 1e-12, else its half-system inverses are stored;
  * CHF_OMEGA_INVERSES=stored in the environment selects the stored inverses.
  * The wall-normal transforms (padded synthesis, analysis, Gauss-Lobatto
- * point values) run as FFTs along y, in the form's own arithmetic, when 2N is one of 64, 96, 128, 192, 256, 384, 512 and
+ * point values, the solves' right-hand sides) run as FFTs along y, in the
+ * form's own arithmetic, when 2N is one of 64, 96, 128, 192, 256, 384, 512 and
  * nnx/2 * nny is a multiple of 16, else as cos-table products on the matrix
  * cores; CHF_Y_TRANSFORMS=matrix selects the products. */
 int chf_create(const chf_params* prm, void** ctx);
