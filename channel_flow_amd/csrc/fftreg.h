@@ -71,6 +71,20 @@ struct C2<float> {
   static __device__ __forceinline__ float2 make(float a, float b) { return make_float2(a, b); }
 };
 
+// Stores marked non-temporal, for outputs that the next pass reads and this
+// one does not: streaming them past L2 leaves it to the rows the pass itself
+// reads again.
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+  typedef double e2 __attribute__((ext_vector_type(2)));
+  const e2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<e2*>(p));
+}
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+  typedef float e2 __attribute__((ext_vector_type(2)));
+  const e2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<e2*>(p));
+}
+
 template <class V>
 __host__ __device__ __forceinline__ V cadd(V a, V b) {
   a.x += b.x;

@@ -202,7 +202,7 @@ __global__ __launch_bounds__(YT<V>::TX* ymax(A1, A2)) void k_ycheb(YArgs a) {
   const TI* in = static_cast<const TI*>(a.in[blockIdx.z]);
   TO* out = static_cast<TO*>(a.out[blockIdx.z]);
   auto ld = [&](int r) { return ycvt<V>(in[(long)r * nc + col]); };
-  auto st = [&](int r, V v) { out[(long)r * nc + col] = ycvt<TO>(v); };
+  auto st = [&](int r, V v) { st_nt(out + (long)r * nc + col, ycvt<TO>(v)); };
   // H_q of the analysis from its two points
   auto hodd = [&](int q, V f, V g) { return q == 0 ? ysub(f, g) : yscale(ysub(f, g), (T)2); };
 
