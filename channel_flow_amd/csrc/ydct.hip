@@ -327,14 +327,33 @@ __global__ __launch_bounds__(YT<double2>::TX* ymax(N1, N2)) void k_yrhs(YRhsArgs
   double2* buf = reinterpret_cast<double2*>(smraw) + tx * SB;
   const long col = xcd_order(blockIdx.x, gridDim.x) * TX + tx;
   const long nc = a.ncol;
+  // Row pairs (p, N - p), p = r + POFF, r = t + i G, of this thread: their
+  // h^n and h^{n-1} are requested as soon as step A has used its inputs, so
+  // they arrive during the exchange and step B instead of after them.
+  constexpr int RI = (N / 2 + G - 1) / G;
+  const int Me = a.ne - POFF, Mo = a.no - POFF;
+  const double c1 = 1.5 * a.dt, c0 = 0.5 * a.dt;
+  double2 x[N2];
   if (t < N1) {
-    double2 x[N2];
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) {
       const int m = n2 * N1 + t;
       x[n2] = a.g[(long)(m <= N ? m : L - m) * nc + col];
     }
-    ystepA<N1, N2, -1>(x, buf, a.twL, t);
+  }
+  if (t < N1) ystepA<N1, N2, -1>(x, buf, a.twL, t);
+  double2 hn[2][RI], hp[2][RI];
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const int r = t + i * G;
+    const int p = r + POFF, q = N - p;
+    hn[0][i] = hp[0][i] = hn[1][i] = hp[1][i] = make_double2(0.0, 0.0);
+    if (r < Me) {
+      hn[0][i] = a.hn[(long)p * nc + col];
+      hp[0][i] = a.hp[(long)p * nc + col];
+      hn[1][i] = a.hn[(long)q * nc + col];
+      hp[1][i] = a.hp[(long)q * nc + col];
+    }
   }
   __syncthreads();
   double2 b[N1];
@@ -349,21 +368,27 @@ __global__ __launch_bounds__(YT<double2>::TX* ymax(N1, N2)) void k_yrhs(YRhsArgs
     }
   }
   __syncthreads();
-  const double c1 = 1.5 * a.dt, c0 = 0.5 * a.dt;
-  auto fF = [&](int j) {
-    const double2 hn = a.hn[(long)j * nc + col], hp = a.hp[(long)j * nc + col];
+  double2 hf[2][RI];
+#pragma unroll
+  for (int i = 0; i < RI; ++i)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+      hf[s][i] = make_double2(c1 * hn[s][i].x - c0 * hp[s][i].x, c1 * hn[s][i].y - c0 * hp[s][i].y);
+  auto fF = [&](int j, double2 h) {
     const double2 f = buf[j];
-    return make_double2(f.x + (c1 * hn.x - c0 * hp.x), f.y + (c1 * hn.y - c0 * hp.y));
+    return make_double2(f.x + h.x, f.y + h.y);
   };
-  const int Me = a.ne - POFF, Mo = a.no - POFF;
-  for (int r = t; r < Me; r += G) {
+#pragma unroll
+  for (int i = 0; i < RI; ++i) {
+    const int r = t + i * G;
+    if (r >= Me) break;
     const int p = r + POFF, q = N - p;
-    const double2 vp = fF(p);
+    const double2 vp = fF(p, hf[0][i]);
     double2 ev, od = make_double2(0.0, 0.0);
     if (q == p) {
       ev = vp;
     } else {
-      const double2 vq = fF(q);
+      const double2 vq = fF(q, hf[1][i]);
       ev = yscale(yadd(vp, vq), 0.5);
       od = yscale(ysub(vp, vq), 0.5);
     }
@@ -473,6 +498,8 @@ hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hn, c
   a.no = c.d.no;
   a.dt = c.prm.dt;
   a.twL = c.twy;
+  // each thread of k_yrhs holds ceil((N/2) / G) row pairs of a parity system
+  if (c.d.ne - 1 > c.d.N / 2 || c.d.no - 1 > c.d.N / 2) return hipErrorNotSupported;
   switch (2 * c.d.N) {
     case 64: return run_yrhs<8, 8>(c, which, a);
     case 96: return run_yrhs<8, 12>(c, which, a);
