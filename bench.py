@@ -56,8 +56,11 @@ def kernel_work(d, precision="fp64", yfft=False):
     FP64 in the FP64 form and FP32 (8 B) in the mixed form.  Flops count the
     useful multiply-adds (2 flops each) of the parity-split contractions.
     With yfft (the wall-normal transforms run as FFTs along y) the padded
-    synthesis, the analysis and the point values have O(N log N) work per
-    column and are priced by their bytes alone.
+    synthesis, the analysis, the point values and the right-hand sides have
+    O(N log N) work per column and are priced by their bytes alone, and the
+    right-hand-side kernels form h^n's point values themselves (they read
+    h^n's coefficients and h^(n-1)'s point values and write h^n's): there is
+    no separate point-value pass.
     """
     w = 16 if precision == "fp64" else 8
     S = d["Ny"] * d["ncol"] * 16            # spectral field (FP64)
@@ -78,10 +81,10 @@ def kernel_work(d, precision="fp64", yfft=False):
         "analysis": (3 * (Y + S), yf * 2.0 * Ny * Ph * col * 3),
         "hassemble": (5 * S + 4 * Ny * 16, 0.0),   # plus the mean-mode point block
         "synth_points": (4 * S, yf * 2.0 * Pn * Ny * col * 2),
-        "rhs_v": (S + 2 * (Ny - 4) / Ny * S + rv_rows / Ny * S,
-                  2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
-        "rhs_o": (S + 2 * (Ny - 2) / Ny * S + ro_rows / Ny * S,
-                  2.0 * ((ne - 1) * ne + (no - 1) * no) * col),
+        "rhs_v": ((4 * S if yfft else S + 2 * (Ny - 4) / Ny * S) + rv_rows / Ny * S,
+                  yf * 2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
+        "rhs_o": ((4 * S if yfft else S + 2 * (Ny - 2) / Ny * S) + ro_rows / Ny * S,
+                  yf * 2.0 * ((ne - 1) * ne + (no - 1) * no) * col),
         # v solve (omega_y of the mean group only) and the diagonalised omega_y solve
         "solve": (inv_v_bytes + rv_rows / Ny * S + S, 0.0),
         "wsolve": (ro_rows / Ny * S + S, ws_flops),

@@ -624,7 +624,22 @@ hipError_t timed(Ctx* c, int id, F&& launch) {
   return hipSuccess;
 }
 
-int nonlinear(Ctx* c) {
+// Forms h^n's point values that nonlinear() left to the next solve, with the
+// FP64 point-value kernel in both forms: the values the right-hand-side
+// kernels would have formed, bit for bit.
+hipError_t form_h_points(Ctx* c) {
+  if (!c->hdefer) return hipSuccess;
+  const double2* hc[2] = {c->hvc, c->hgc};
+  double2* ho[2] = {c->hv[c->hcur], c->hg[c->hcur]};
+  const hipError_t e = launch_ypoints(*c, hc, ho, 2);
+  if (e == hipSuccess) c->hdefer = false;
+  return e;
+}
+
+// defer: with the FFT transforms, leave h^n's point values to the next
+// solve's right-hand-side kernels (the ensemble's graphs form them here).
+int nonlinear(Ctx* c, bool defer = true) {
+  CHF_TRY(form_h_points(c));  // the previous step's, before the h assembly overwrites hvc, hgc
   // six fields in the order the streamwise pass pairs them: (u,v) (w,ox) (oy,oz)
   const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
   if (c->prm.precision == 0) {
@@ -647,9 +662,13 @@ int nonlinear(Ctx* c) {
   }
   CHF_TRY(timed(c, K_HASSEMBLE, [&] { return launch_hassemble(*c); }));
   const int nxt = 1 - c->hcur;
-  const double2* hc[2] = {c->hvc, c->hgc};
-  double2* ho[2] = {c->hv[nxt], c->hg[nxt]};
-  CHF_TRY(timed(c, K_SYNTH_POINTS, [&] { return launch_synth_points_h(*c, hc, ho, 2); }));
+  if (defer && c->twy) {
+    c->hdefer = true;
+  } else {
+    const double2* hc[2] = {c->hvc, c->hgc};
+    double2* ho[2] = {c->hv[nxt], c->hg[nxt]};
+    CHF_TRY(timed(c, K_SYNTH_POINTS, [&] { return launch_synth_points_h(*c, hc, ho, 2); }));
+  }
   c->hcur = nxt;
   return 0;
 }
@@ -670,11 +689,13 @@ int solve(Ctx* c) {
     c->stream = main;
     CHF_TRY(e);
     CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
+    c->hdefer = false;  // both right-hand-side kernels have formed h^n's point values
     CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c); }));
     CHF_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   } else {
     CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
     CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
+    c->hdefer = false;
     CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c); }));
     if (c->wdiag) CHF_TRY(timed(c, K_WSOLVE, [&] { return launch_wsolve(*c); }));
   }
@@ -689,6 +710,7 @@ int solve(Ctx* c) {
 int solve_before(Ctx* c) {
   CHF_TRY(timed(c, K_RHS_V, [&] { return launch_rhs(*c, 0); }));
   CHF_TRY(timed(c, K_RHS_O, [&] { return launch_rhs(*c, 1); }));
+  c->hdefer = false;
   if (c->wdiag) CHF_TRY(timed(c, K_WSOLVE, [&] { return launch_wsolve(*c); }));
   return 0;
 }
@@ -776,6 +798,7 @@ int chf_set_state(void* ctx, const chf_state* st) {
   for (auto& e : m)
     if (!e.src) return fail(c, "chf_set_state: null array");
   for (auto& e : m) CHF_TRY(hipMemcpyAsync(e.dst, e.src, e.bytes, hipMemcpyHostToDevice, c->stream));
+  c->hdefer = false;  // h^n's point values are the record's
   // u, w, omega_x, omega_z and the brackets of the loaded v, omega_y, U, W
   CHF_TRY(load_kinematics(c));
   return check_finite(c, "chf_set_state");
@@ -796,6 +819,7 @@ int chf_get_state(void* ctx, const chf_state* st) {
       {c->hv[pv], st->unp1, fb}, {c->hg[pv], st->wnp1, fb}};
   for (auto& e : m)
     if (!e.dst) return fail(c, "chf_get_state: null array");
+  CHF_TRY(form_h_points(c));
   for (auto& e : m) CHF_TRY(hipMemcpyAsync(e.dst, e.src, e.bytes, hipMemcpyDeviceToHost, c->stream));
   CHF_TRY(hipStreamSynchronize(c->stream));
   return 0;
@@ -813,6 +837,7 @@ int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1, cons
   CHF_TRY(hipMemcpyAsync(c->au0, au0, mb, hipMemcpyHostToDevice, c->stream));
   CHF_TRY(hipMemcpyAsync(c->w0, w0, mb, hipMemcpyHostToDevice, c->stream));
   // h^n = 0 and L^n = 0 so that the rotation in getnonlinear leaves h^{-1} = 0 (PF:2316)
+  c->hdefer = false;
   for (double2* q : {c->hv[0], c->hv[1], c->hg[0], c->hg[1]})
     CHF_TRY(hipMemsetAsync(q, 0, fb, c->stream));
   for (double2* q : {c->cl1[0], c->cl1[1], c->cl3[0], c->cl3[1]})
@@ -1110,7 +1135,7 @@ int ens_capture(Ens* e, EnsGraph* g) {
     he = hipStreamWaitEvent(c->stream, e->mid, 0);
     if (he != hipSuccess) break;
     rc = solve_after(c);
-    if (rc == 0) rc = nonlinear(c);
+    if (rc == 0) rc = nonlinear(c, false);
     if (rc) {
       what = c->err;
       break;
@@ -1222,6 +1247,8 @@ int chf_ensemble_step(void* ens, int n) {
   if (n < 0) return ens_fail(e, "chf_ensemble_step: negative step count");
   for (Ctx* c : e->m) {
     if (c->timing) return ens_fail(e, "chf_ensemble_step: turn member timing off first");
+    // the graphs form h^n's point values in nonlinear(): pending ones first
+    ENS_TRY(form_h_points(c));
     // member work queued outside the graph (set_state, init) must finish first
     ENS_TRY(hipStreamSynchronize(c->stream));
   }

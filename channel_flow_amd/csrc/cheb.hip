@@ -160,15 +160,16 @@ hipError_t launch_synth_points(Ctx& c, const double2* const* in, double2* const*
 }
 
 // h_v, h_g at the N+1 points (the AB2 forcing history): FP64 arrays in both
-// forms, computed in FP32 arithmetic in the mixed form (FFTs, or the FP32
-// matrix cores for degrees without an FFT split).  The values
-// enter the solve only through dt (3 h^n - h^{n-1}) / 2, as the FP32 Lamb
-// vector they are formed from does (measured at 256x193x256 over 100 steps,
-// against the FP64 form: field errors within 3% of those with FP64 point
-// values, wall shear 3.1e-10 / 1.2e-9 against 3.6e-10 / 1.0e-9).
+// forms.  With the FFT transforms they are FP64 arithmetic in both forms, as
+// the right-hand-side kernels that form them there (a run that forms them in
+// either place stays bit-identical).  With the cos-table products the mixed
+// form computes them on the FP32 matrix cores: the values enter the solve only
+// through dt (3 h^n - h^{n-1}) / 2, as the FP32 Lamb vector they are formed
+// from does (measured at 256x193x256 over 100 steps, against the FP64 form:
+// field errors within 3% of those with FP64 point values, wall shear
+// 3.1e-10 / 1.2e-9 against 3.6e-10 / 1.0e-9).
 hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  if (c.prm.precision == 0) return launch_synth_points(c, in, out, nf);
-  if (c.twy) return launch_ypoints32(c, in, out, nf);
+  if (c.prm.precision == 0 || c.twy) return launch_synth_points(c, in, out, nf);
   const double* ri[6];
   double* ro[6];
   as_real(in, ri, nf);
@@ -402,9 +403,10 @@ static hipError_t rhs_syn(Ctx& c, const double2* g, const double2* hn, const dou
 hipError_t launch_rhs(Ctx& c, int which) {
   const int nxt = c.hcur, prv = 1 - c.hcur;  // h^n, h^{n-1}
   if (c.twy) {
-    // FFT form (ydct.hip): the bracket's point values by the type-I DCT
-    if (which == 0) return launch_yrhs(c, 0, c.gv, c.hv[nxt], c.hv[prv], nullptr, nullptr, c.rvT);
-    return launch_yrhs(c, 1, c.go, c.hg[nxt], c.hg[prv], c.roE, c.roO, nullptr);
+    // FFT form (ydct.hip): the bracket's point values by the type-I DCT, and
+    // h^n's from its coefficients when they are not formed yet
+    if (which == 0) return launch_yrhs(c, 0, c.gv, c.hdefer ? c.hvc : nullptr, c.hv[nxt], c.hv[prv], nullptr, nullptr, c.rvT);
+    return launch_yrhs(c, 1, c.go, c.hdefer ? c.hgc : nullptr, c.hg[nxt], c.hg[prv], c.roE, c.roO, nullptr);
   }
   if (which == 0) return rhs_syn<2>(c, c.gv, c.hv[nxt], c.hv[prv], nullptr, nullptr, c.rvT);
   return rhs_syn<1>(c, c.go, c.hg[nxt], c.hg[prv], c.roE, c.roO, nullptr);

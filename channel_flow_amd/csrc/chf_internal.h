@@ -107,6 +107,10 @@ struct Ctx {
   double2* cl3[2] = {nullptr, nullptr};  // mean L_z(0,0) point values [Ny]
   double2 *au0 = nullptr, *w0 = nullptr;  // mean U, W coefficients [Ny]
   int hcur = 0;
+  // h^n's point values hv[hcur], hg[hcur] are not yet formed: hvc, hgc hold
+  // h^n's coefficients and the next solve's right-hand-side kernels form them
+  // (FFT transforms only)
+  bool hdefer = false;
   // nonlinear work
   double2* Y[6] = {};     // [Py][Mz][Mx]   wall-normal points, Fourier modes
   double2* Z[6] = {};     // [Py][Lz][Mx]   wall-normal + span points, x modes
@@ -216,9 +220,10 @@ int ydct_length_supported(int N);  // 1 when 2N has an FFT split
 hipError_t launch_ypoints(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_ysynth(Ctx& c, const double2* const* in, double2* const* out, int nf);
 hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* out, int nf);
-hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hn, const double2* hp, double2* oE,
-                       double2* oO, double2* oT);  // which: 0 = v, 1 = omega_y
-hipError_t launch_ypoints32(Ctx& c, const double2* const* in, double2* const* out, int nf);   // FP32 arithmetic
+// which: 0 = v, 1 = omega_y; hc != nullptr: h^n from its coefficients hc, its
+// point values written to hn; else hn holds them
+hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hc, double2* hn, const double2* hp,
+                       double2* oE, double2* oO, double2* oT);
 hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf);
 hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf);  // FP32 compute in the mixed form

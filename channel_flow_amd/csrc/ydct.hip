@@ -304,12 +304,22 @@ __global__ __launch_bounds__(YT<V>::TX* ymax(A1, A2)) void k_ycheb(YArgs a) {
 // DCT, FP64), plus the AB2 forcing F = 3 dt/2 h^n - dt/2 h^(n-1), folded by
 // parity at the point pairs p = r + POFF, q = N - p: even row r
 // ((f+F)_p + (f+F)_q)/2 (rows r < ne - POFF), odd row r ((f+F)_p - (f+F)_q)/2
-// (r < no - POFF).  The point values pass through the LDS tile so that each
-// thread sees both points of its pairs.  v: each mode's rows as one record,
-// [column][Rv: even rows, then odd rows]; omega_y: [row][column] per parity.
+// (r < no - POFF).  v: each mode's rows as one record, [column][Rv: even rows,
+// then odd rows]; omega_y: [row][column] per parity.
+//
+// FUSE: h^n arrives as coefficients (the h assembly's output) and this kernel
+// forms its point values, writes them (the next step's h^(n-1) and the state
+// record's h^n) and uses them, in place of a separate point-value pass and a
+// read of its output.  Otherwise h^n's point values are read.  Both forms
+// compute every value the same way (the point values with the same FFT and
+// end terms as k_ypoints, f + F in the step-B layout, the pairs from the LDS
+// tile), so a run that switches between them is bit-identical to one that
+// does not.
 struct YRhsArgs {
   const double2* g;
-  const double2 *hn, *hp;
+  const double2* hc;  // FUSE: h^n coefficients
+  double2* hn;        // h^n point values (FUSE: written, else read)
+  const double2* hp;  // h^(n-1) point values
   double2 *outT, *outE, *outO;
   long ncol;
   int Rv, ne, no;
@@ -317,78 +327,109 @@ struct YRhsArgs {
   const double2* twL;
 };
 
-template <int N1, int N2, int POFF>
+template <int N1, int N2, int POFF, bool FUSE>
 __global__ __launch_bounds__(YT<double2>::TX* ymax(N1, N2)) void k_yrhs(YRhsArgs a) {
   constexpr int L = N1 * N2, N = L / 2, TX = YT<double2>::TX, G = ymax(N1, N2);
   constexpr int SB = tile_stride<double2>(N1 * (N2 + 1));
+  constexpr int KJ = N / N2 + 1;  // step-B outputs k1 with a point j = k1 N2 + t <= N
   static_assert(SB > N, "the point values fit the exchange tile");
   extern __shared__ __align__(16) unsigned char smraw[];
   const int t = threadIdx.x / TX, tx = threadIdx.x - t * TX;
   double2* buf = reinterpret_cast<double2*>(smraw) + tx * SB;
   const long col = xcd_order(blockIdx.x, gridDim.x) * TX + tx;
   const long nc = a.ncol;
-  // Row pairs (p, N - p), p = r + POFF, r = t + i G, of this thread: their
-  // h^n and h^{n-1} are requested as soon as step A has used its inputs, so
-  // they arrive during the exchange and step B instead of after them.
-  constexpr int RI = (N / 2 + G - 1) / G;
-  const int Me = a.ne - POFF, Mo = a.no - POFF;
   const double c1 = 1.5 * a.dt, c0 = 0.5 * a.dt;
-  double2 x[N2];
-  if (t < N1) {
+  // step A of the type-I DCT of the coefficient column src (threads t < N1)
+  auto stepA = [&](const double2* src) {
+    if (t < N1) {
+      double2 x[N2];
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) {
-      const int m = n2 * N1 + t;
-      x[n2] = a.g[(long)(m <= N ? m : L - m) * nc + col];
+      for (int n2 = 0; n2 < N2; ++n2) {
+        const int m = n2 * N1 + t;
+        x[n2] = src[(long)(m <= N ? m : L - m) * nc + col];
+      }
+      ystepA<N1, N2, -1>(x, buf, a.twL, t);
     }
-  }
-  if (t < N1) ystepA<N1, N2, -1>(x, buf, a.twL, t);
-  double2 hn[2][RI], hp[2][RI];
-#pragma unroll
-  for (int i = 0; i < RI; ++i) {
-    const int r = t + i * G;
-    const int p = r + POFF, q = N - p;
-    hn[0][i] = hp[0][i] = hn[1][i] = hp[1][i] = make_double2(0.0, 0.0);
-    if (r < Me) {
-      hn[0][i] = a.hn[(long)p * nc + col];
-      hp[0][i] = a.hp[(long)p * nc + col];
-      hn[1][i] = a.hn[(long)q * nc + col];
-      hp[1][i] = a.hp[(long)q * nc + col];
-    }
-  }
-  __syncthreads();
-  double2 b[N1];
-  if (t < N2) fft_stepB<N1, N2, -1>(b, buf, t);
-  __syncthreads();  // the tile now takes the point values
-  if (t < N2) {
-    const double2 g0 = a.g[col], gN = a.g[(long)N * nc + col];
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) {
-      const int j = k1 * N2 + t;
-      if (j <= N) buf[j] = yscale(yadd(yadd(b[k1], g0), (j & 1) ? yscale(gN, -1.0) : gN), 0.5);
-    }
-  }
-  __syncthreads();
-  double2 hf[2][RI];
-#pragma unroll
-  for (int i = 0; i < RI; ++i)
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-      hf[s][i] = make_double2(c1 * hn[s][i].x - c0 * hp[s][i].x, c1 * hn[s][i].y - c0 * hp[s][i].y);
-  auto fF = [&](int j, double2 h) {
-    const double2 f = buf[j];
-    return make_double2(f.x + h.x, f.y + h.y);
   };
+  // f_j = (S_j + a_0 + (-1)^j a_N) / 2 at j = k1 N2 + t, after step B
+  auto endterms = [&](const double2 (&b)[N1], const double2* src, double2 (&f)[KJ]) {
+    const double2 s0 = src[col], sN = src[(long)N * nc + col];
 #pragma unroll
-  for (int i = 0; i < RI; ++i) {
-    const int r = t + i * G;
-    if (r >= Me) break;
+    for (int k1 = 0; k1 < KJ; ++k1) {
+      const int j = k1 * N2 + t;
+      f[k1] = yscale(yadd(yadd(b[k1], s0), (j & 1) ? yscale(sN, -1.0) : sN), 0.5);
+    }
+  };
+  // AB2 forcing c1 h^n - c0 h^(n-1) at one point
+  auto ab2 = [&](double2 n, double2 p) { return make_double2(c1 * n.x - c0 * p.x, c1 * n.y - c0 * p.y); };
+  double2 hf[KJ], hn[KJ], hp[KJ];  // at this thread's step-B points
+  if constexpr (FUSE) {
+    stepA(a.hc);
+    if (t < N2) {
+#pragma unroll
+      for (int k1 = 0; k1 < KJ; ++k1) {
+        const int j = k1 * N2 + t;
+        if (j <= N) hp[k1] = a.hp[(long)j * nc + col];
+      }
+    }
+    __syncthreads();
+    if (t < N2) {
+      double2 b[N1];
+      fft_stepB<N1, N2, -1>(b, buf, t);
+      endterms(b, a.hc, hn);
+#pragma unroll
+      for (int k1 = 0; k1 < KJ; ++k1) {
+        const int j = k1 * N2 + t;
+        if (j <= N) {
+          st_nt(a.hn + (long)j * nc + col, hn[k1]);
+          hf[k1] = ab2(hn[k1], hp[k1]);
+        }
+      }
+    }
+    __syncthreads();  // the tile is reused
+  }
+  stepA(a.g);
+  if constexpr (!FUSE) {
+    // requested before the exchange, so that they arrive during it
+    if (t < N2) {
+#pragma unroll
+      for (int k1 = 0; k1 < KJ; ++k1) {
+        const int j = k1 * N2 + t;
+        if (j <= N) {
+          hn[k1] = a.hn[(long)j * nc + col];
+          hp[k1] = a.hp[(long)j * nc + col];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double2 f[KJ];
+  if (t < N2) {
+    double2 b[N1];
+    fft_stepB<N1, N2, -1>(b, buf, t);
+    endterms(b, a.g, f);
+  }
+  __syncthreads();  // the tile now takes the values f + F
+  if (t < N2) {
+#pragma unroll
+    for (int k1 = 0; k1 < KJ; ++k1) {
+      const int j = k1 * N2 + t;
+      if (j <= N) {
+        const double2 h = FUSE ? hf[k1] : ab2(hn[k1], hp[k1]);
+        buf[j] = make_double2(f[k1].x + h.x, f[k1].y + h.y);
+      }
+    }
+  }
+  __syncthreads();
+  const int Me = a.ne - POFF, Mo = a.no - POFF;
+  for (int r = t; r < Me; r += G) {
     const int p = r + POFF, q = N - p;
-    const double2 vp = fF(p, hf[0][i]);
+    const double2 vp = buf[p];
     double2 ev, od = make_double2(0.0, 0.0);
     if (q == p) {
       ev = vp;
     } else {
-      const double2 vq = fF(q, hf[1][i]);
+      const double2 vq = buf[q];
       ev = yscale(yadd(vp, vq), 0.5);
       od = yscale(ysub(vp, vq), 0.5);
     }
@@ -428,7 +469,11 @@ template <int A1, int A2, class V, class TI, class TO>
 static hipError_t run_y(Ctx& c, int mode, const YArgs& a, int nf) {
   constexpr int TX = YT<V>::TX, nt = TX * ymax(A1, A2);
   const size_t shm = sizeof(V) * (size_t)TX * tile_stride<V>(A1 * (A2 + 1));
-  if (mode == 0) return launch_y(c, k_ypoints<A1, A2, V>, TX, nt, shm, a, nf);
+  if (mode == 0) {
+    // point values are FP64 arithmetic only (h^n, the statistics profiles)
+    if constexpr (sizeof(V) == sizeof(double2)) return launch_y(c, k_ypoints<A1, A2, V>, TX, nt, shm, a, nf);
+    return hipErrorNotSupported;
+  }
   if (mode == 1) return launch_y(c, k_ycheb<A1, A2, V, TI, TO, false, single_load<V, false>()>, TX, nt, shm, a, nf);
   return launch_y(c, k_ycheb<A1, A2, V, TI, TO, true, single_load<V, true>()>, TX, nt, shm, a, nf);
 }
@@ -476,17 +521,21 @@ hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* ou
   return run_ydct<double2>(c, 2, in, out, nf);
 }
 template <int N1, int N2>
-static hipError_t run_yrhs(Ctx& c, int which, const YRhsArgs& a) {
+static hipError_t run_yrhs(Ctx& c, int which, bool fuse, const YRhsArgs& a) {
   constexpr int TX = YT<double2>::TX, nt = TX * ymax(N1, N2);
   const size_t shm = sizeof(double2) * (size_t)TX * tile_stride<double2>(N1 * (N2 + 1));
-  if (which == 0) return launch_y(c, k_yrhs<N1, N2, 2>, TX, nt, shm, a, 1);
-  return launch_y(c, k_yrhs<N1, N2, 1>, TX, nt, shm, a, 1);
+  if (which == 0)
+    return fuse ? launch_y(c, k_yrhs<N1, N2, 2, true>, TX, nt, shm, a, 1)
+                : launch_y(c, k_yrhs<N1, N2, 2, false>, TX, nt, shm, a, 1);
+  return fuse ? launch_y(c, k_yrhs<N1, N2, 1, true>, TX, nt, shm, a, 1)
+              : launch_y(c, k_yrhs<N1, N2, 1, false>, TX, nt, shm, a, 1);
 }
 
-hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hn, const double2* hp, double2* oE,
-                       double2* oO, double2* oT) {
+hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hc, double2* hn, const double2* hp,
+                       double2* oE, double2* oO, double2* oT) {
   YRhsArgs a{};
   a.g = g;
+  a.hc = hc;
   a.hn = hn;
   a.hp = hp;
   a.outT = oT;
@@ -498,23 +547,19 @@ hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hn, c
   a.no = c.d.no;
   a.dt = c.prm.dt;
   a.twL = c.twy;
-  // each thread of k_yrhs holds ceil((N/2) / G) row pairs of a parity system
-  if (c.d.ne - 1 > c.d.N / 2 || c.d.no - 1 > c.d.N / 2) return hipErrorNotSupported;
+  const bool fuse = hc != nullptr;
   switch (2 * c.d.N) {
-    case 64: return run_yrhs<8, 8>(c, which, a);
-    case 96: return run_yrhs<8, 12>(c, which, a);
-    case 128: return run_yrhs<8, 16>(c, which, a);
-    case 192: return run_yrhs<12, 16>(c, which, a);
-    case 256: return run_yrhs<16, 16>(c, which, a);
-    case 384: return run_yrhs<16, 24>(c, which, a);
-    case 512: return run_yrhs<16, 32>(c, which, a);
+    case 64: return run_yrhs<8, 8>(c, which, fuse, a);
+    case 96: return run_yrhs<8, 12>(c, which, fuse, a);
+    case 128: return run_yrhs<8, 16>(c, which, fuse, a);
+    case 192: return run_yrhs<12, 16>(c, which, fuse, a);
+    case 256: return run_yrhs<16, 16>(c, which, fuse, a);
+    case 384: return run_yrhs<16, 24>(c, which, fuse, a);
+    case 512: return run_yrhs<16, 32>(c, which, fuse, a);
     default: return hipErrorNotSupported;
   }
 }
 
-hipError_t launch_ypoints32(Ctx& c, const double2* const* in, double2* const* out, int nf) {
-  return run_ydct<float2>(c, 0, in, out, nf);
-}
 hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf) {
   return run_ydct<float2>(c, 1, in, out, nf);
 }
