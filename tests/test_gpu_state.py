@@ -123,10 +123,14 @@ def test_set_state_derives_the_fields(require_gpu):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_resume_is_bit_identical(require_gpu, precision):
+@pytest.mark.parametrize("nnz", [24, 32])
+def test_resume_is_bit_identical(require_gpu, precision, nnz):
     """20 straight steps against 10 steps, chf_get_state, a new context,
-    chf_set_state and 10 more steps: every state array and field identical."""
-    cfg = dict(nnx=32, nny=32, nnz=24, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0, dt=1.0e-3,
+    chf_set_state and 10 more steps: every state array and field identical.
+    N = 24 runs the wall-normal transforms as cos-table products, N = 32 as
+    FFTs (the resumed run's first right-hand sides then read h^n's point
+    values from the record, where the straight run forms them)."""
+    cfg = dict(nnx=32, nny=32, nnz=nnz, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0, dt=1.0e-3,
                u_c=1.0, rms=0.1, kmax=8.0, slope=-4.0 / 3.0, seed=41)
     arrays = ic_of(cfg)
     with make_channel(cfg, precision) as a:
@@ -145,6 +149,59 @@ def test_resume_is_bit_identical(require_gpu, precision):
         np.testing.assert_array_equal(resumed[k], straight[k], err_msg=f"{precision} {k}")
     for k in fs:
         np.testing.assert_array_equal(fr[k], fs[k], err_msg=f"{precision} {k}")
+
+
+def test_h_point_values_whatever_the_call_order(require_gpu, monkeypatch):
+    """With the FFT transforms the next solve's right-hand-side kernels form
+    h^n's point values, or chf_get_state or a second nonlinear pass forms them
+    first.  (1) A run split by state reads and separate solve / nonlinear calls
+    is bit-identical to chf_step straight through.  (2) Two nonlinear passes in
+    a row (the AB2 history then holds the same h twice) leave the state that
+    the cos-table transforms, which form the point values in the nonlinear
+    pass, leave (FP64, 1e-11)."""
+    cfg = dict(nnx=32, nny=32, nnz=32, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0, dt=1.0e-3,
+               u_c=1.0, rms=0.1, kmax=8.0, slope=-4.0 / 3.0, seed=43)
+    arrays = ic_of(cfg)
+    monkeypatch.delenv("CHF_Y_TRANSFORMS", raising=False)
+    with make_channel(cfg) as a:
+        assert a.tile_choices() == {}, "the FFT transforms are expected at this size"
+        a.initial(*arrays)
+        a.step(6)
+        straight, fs = a.get_state(), a.fields()
+    with make_channel(cfg) as b:
+        b.initial(*arrays)
+        b.get_state()                                   # before the first solve
+        b.step(2)
+        b.solve()
+        b.get_state()                                   # between a solve and its nonlinear pass
+        b.getnonlinear()
+        b.step(1)
+        b.solve()
+        b.getnonlinear()
+        b.get_state()
+        b.step(1)
+        split, fsp = b.get_state(), b.fields()
+    for k in STATE_KEYS:
+        np.testing.assert_array_equal(split[k], straight[k], err_msg=k)
+    for k in fs:
+        np.testing.assert_array_equal(fsp[k], fs[k], err_msg=k)
+
+    def twice(matrix):
+        if matrix:
+            monkeypatch.setenv("CHF_Y_TRANSFORMS", "matrix")
+        else:
+            monkeypatch.delenv("CHF_Y_TRANSFORMS", raising=False)
+        with make_channel(cfg) as c:
+            c.initial(*arrays)
+            c.step(2)
+            c.getnonlinear()
+            c.step(1)
+            return c.get_state()
+
+    errs = state_errors(twice(False), twice(True))
+    report("two nonlinear passes, FFT vs cos-table transforms", errs)
+    for k, e in errs.items():
+        assert e < 1e-11, (k, e)
 
 
 def test_nonfinite_state_is_reported(require_gpu):
