@@ -77,7 +77,10 @@ const char* chf_last_error(void* ctx);
  * resumed from a chf_get_state record continues bit for bit as the
  * uninterrupted run does. */
 int chf_set_state(void* ctx, const chf_state* st);
-/* Device -> host copy of the pre-solve state (replaces output, PF:2460-2504). */
+/* Device -> host copy of the pre-solve state (replaces output, PF:2460-2504).
+ * With the FFT transforms, first forms h^n's point values when the last
+ * chf_nonlinear left them to the next solve (the values that solve would
+ * form, bit for bit). */
 int chf_get_state(void* ctx, const chf_state* st);
 
 /* Cold start (replaces initial(isign=-1) after input, PF:1932-1946): from v,
@@ -88,7 +91,10 @@ int chf_init_fields(void* ctx, const double* vnp1, const double* omegaynp1,
                     const double* au0, const double* w0);
 
 /* getnonlinear (PF:1380-1612): six dealiased products -> Lamb vector ->
- * h_v, h_g, AB2 history rotation, point values. */
+ * h_v, h_g, AB2 history rotation, point values.  With the FFT transforms the
+ * point values of h^n are formed by the next chf_solve's right-hand-side
+ * kernels (or by chf_get_state, or a second chf_nonlinear, if one comes
+ * first); results are the same either way. */
 int chf_nonlinear(void* ctx);
 /* Per-mode v / omega_y solves, kinematics and the mean mode (PF:242-474).
  * Returns after the solve has completed (CHF_NONFINITE on a blow-up). */
