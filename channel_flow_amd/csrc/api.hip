@@ -636,13 +636,55 @@ hipError_t form_h_points(Ctx* c) {
   return e;
 }
 
+// Runs a() on the context's stream and b() on the side stream, both after the
+// work queued so far, and joins them: two field groups of one transform
+// stage, so that each launch's tail overlaps the other's work.
+template <class A, class B>
+hipError_t on_two_streams(Ctx* c, A&& a, B&& b) {
+  hipError_t e = hipEventRecord(c->ev_fork, c->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->ev_fork, 0);
+  if (e == hipSuccess) e = a();
+  if (e != hipSuccess) return e;
+  hipStream_t main = c->stream;
+  c->stream = c->side;  // the launchers use c->stream
+  e = b();
+  if (e == hipSuccess) e = hipEventRecord(c->ev_join, c->side);
+  c->stream = main;
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->ev_join, 0);
+  return e;
+}
+
 // defer: with the FFT transforms, leave h^n's point values to the next
 // solve's right-hand-side kernels (the ensemble's graphs form them here).
 int nonlinear(Ctx* c, bool defer = true) {
   CHF_TRY(form_h_points(c));  // the previous step's, before the h assembly overwrites hvc, hgc
   // six fields in the order the streamwise pass pairs them: (u,v) (w,ox) (oy,oz)
   const double2* in6[6] = {c->u, c->v, c->w, c->ox, c->oy, c->oz};
-  if (c->prm.precision == 0) {
+  if (!c->timing && c->overlap) {
+    // synthesis and span FFTs of fields 0-2 and 3-5, then of L_x, L_y and
+    // L_z, on the two streams (with timing on every launch runs alone)
+    if (c->prm.precision == 0) {
+      CHF_TRY(on_two_streams(
+          c, [&] { const hipError_t e = launch_synth_padded(*c, in6, c->Y, 3); return e ? e : launch_zfft_inv(*c, c->Y, c->Z, 3); },
+          [&] { const hipError_t e = launch_synth_padded(*c, in6 + 3, c->Y + 3, 3); return e ? e : launch_zfft_inv(*c, c->Y + 3, c->Z + 3, 3); }));
+      CHF_TRY(launch_xpass(*c));
+      const double2* y3[3] = {c->Y[0], c->Y[1], c->Y[2]};
+      CHF_TRY(on_two_streams(
+          c, [&] { const hipError_t e = launch_zfft_fwd(*c, c->Z, c->Y, 2); return e ? e : launch_analysis(*c, y3, c->Lc, 2); },
+          [&] { const hipError_t e = launch_zfft_fwd(*c, c->Z + 2, c->Y + 2, 1); return e ? e : launch_analysis(*c, y3 + 2, c->Lc + 2, 1); }));
+    } else {
+      const float2* y6[6] = {c->Yf[0], c->Yf[1], c->Yf[2], c->Yf[3], c->Yf[4], c->Yf[5]};
+      const float2* z3[3] = {c->Zf[0], c->Zf[1], c->Zf[2]};
+      const float2* y3[3] = {c->Yf[0], c->Yf[1], c->Yf[2]};
+      CHF_TRY(on_two_streams(
+          c, [&] { const hipError_t e = launch_synth_padded32(*c, in6, c->Yf, 3); return e ? e : launch_zfft32(*c, y6, c->Zf, 3, true); },
+          [&] { const hipError_t e = launch_synth_padded32(*c, in6 + 3, c->Yf + 3, 3); return e ? e : launch_zfft32(*c, y6 + 3, c->Zf + 3, 3, true); }));
+      CHF_TRY(launch_xpass32(*c));
+      CHF_TRY(on_two_streams(
+          c, [&] { const hipError_t e = launch_zfft32(*c, z3, c->Yf, 2, false); return e ? e : launch_analysis32(*c, y3, c->Lc, 2); },
+          [&] { const hipError_t e = launch_zfft32(*c, z3 + 2, c->Yf + 2, 1, false); return e ? e : launch_analysis32(*c, y3 + 2, c->Lc + 2, 1); }));
+    }
+  } else if (c->prm.precision == 0) {
     CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return launch_synth_padded(*c, in6, c->Y, 6); }));
     CHF_TRY(timed(c, K_ZFFT_INV, [&] { return launch_zfft_inv(*c, c->Y, c->Z, 6); }));
     CHF_TRY(timed(c, K_XPASS, [&] { return launch_xpass(*c); }));
