@@ -14,6 +14,7 @@
 // LDS); a step multiplies the stored inverses, as the reference multiplies the
 // zgetri inverses with a_multiply_x (PF:4123-4139).
 #include "chf_internal.h"
+#include "fftreg.h"
 #include "mfma.h"
 
 namespace chf {
@@ -440,7 +441,9 @@ __global__ __launch_bounds__(512) void k_solve(SolveArgs a1, const SolveArgs* __
 // reads v at degrees of parity q and omega_y at parity 1-q, and produces
 // Dv, u, w at parity 1-q and Du, Dw, omega_x, omega_z at parity q.  One thread
 // runs one chain of one mode, backwards over the degree, with its loads issued
-// eight degrees ahead.
+// eight degrees ahead.  Its outputs are stored non-temporally: the next
+// kernels read them, this one does not (the padded synthesis that reads u, w,
+// omega_x, omega_z runs 5% faster, this pass 3% slower).
 //
 // The same pass forms the explicit-operator brackets of the next solve's
 // right-hand sides (get_coef_rhs / get_rhs_omegay, PF:252-257, 335-340):
@@ -538,8 +541,8 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
           du.x *= 0.5; du.y *= 0.5; dw.x *= 0.5; dw.y *= 0.5;
         }
         const double2 vk = in[s];
-        a.ox[off] = make_double2(dw.x + be * vk.y, dw.y - be * vk.x);
-        a.oz[off] = make_double2(-du.x - al * vk.y, -du.y + al * vk.x);
+        st_nt(&a.ox[off], make_double2(dw.x + be * vk.y, dw.y - be * vk.x));
+        st_nt(&a.oz[off], make_double2(-du.x - al * vk.y, -du.y + al * vk.x));
         // brackets: D^2 v, D^4 v at k from D v, D^3 v at k+1; D omega at k from omega_{k+1}
         double2 q2 = make_double2(d2s.x + f * dv2.x, d2s.y + f * dv2.y);
         double2 q4 = make_double2(d4s.x + f * d3s.x, d4s.y + f * d3s.y);
@@ -549,7 +552,7 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
         if (k == 0) {
           q2.x *= 0.5; q2.y *= 0.5; q4.x *= 0.5; q4.y *= 0.5;
         }
-        a.gv[off] = make_double2(cc * q4.x + c2 * q2.x - c0 * vk.x, cc * q4.y + c2 * q2.y - c0 * vk.y);
+        st_nt(&a.gv[off], make_double2(cc * q4.x + c2 * q2.x - c0 * vk.x, cc * q4.y + c2 * q2.y - c0 * vk.y));
         vp = vk;
       } else {
         // Dv_k from v_{k+1}; u_k, w_k with omega_y_k
@@ -561,8 +564,8 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
         const double omr = in[s].x, omi = in[s].y;
         const double2 uk = make_double2(-1.0 * al * dv.y / k2 + be * omi / k2, al * dv.x / k2 - be * omr / k2);
         const double2 wk = make_double2(-1.0 * dv.y * be / k2 - 1.0 * omi * al / k2, be * dv.x / k2 + al * omr / k2);
-        a.u[off] = uk;
-        a.w[off] = wk;
+        st_nt(&a.u[off], uk);
+        st_nt(&a.w[off], wk);
         up = uk;
         wp = wk;
         // brackets: D^3 v at k from D^2 v at k+1; D^2 omega at k from D omega at k+1
@@ -572,7 +575,7 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
         if (k == 0) {
           r2.x *= 0.5; r2.y *= 0.5;
         }
-        a.go[off] = make_double2(cc * r2.x + cw * omr, cc * r2.y + cw * omi);
+        st_nt(&a.go[off], make_double2(cc * r2.x + cw * omr, cc * r2.y + cw * omi));
         wn = in[s];
       }
       probe = fma(in[s].x, 0.0, fma(in[s].y, 0.0, probe));
