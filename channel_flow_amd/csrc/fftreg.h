@@ -112,6 +112,22 @@ __host__ __device__ __forceinline__ V cmulv(V a, V b) {
   return r;
 }
 
+// a (S i)^q: a twiddle that is a whole number of quarter turns, applied as a
+// swap and sign changes (a product with the exact 0 and +-1 of the table costs
+// four arithmetic instructions, which IEEE semantics keep the compiler from
+// removing, and gives the same values).
+template <int S, class V>
+__host__ __device__ __forceinline__ V quarter_turns(V a, int q) {
+  V r;
+  switch (((S > 0 ? q : -q) % 4 + 4) % 4) {
+    case 0: r = a; break;
+    case 1: r.x = -a.y; r.y = a.x; break;   // i a
+    case 2: r.x = -a.x; r.y = -a.y; break;  // -a
+    default: r.x = a.y; r.y = -a.x; break;  // -i a
+  }
+  return r;
+}
+
 // In-place, natural-order DFT of R points, X[k] = sum_n x[n] e^{S 2 pi i n k / R}.
 template <int R, int S, class V>
 struct Dft;
@@ -189,6 +205,8 @@ struct Dft {
       for (int f = 0; f < F; ++f) {
         if (f == 0 || k == 0) {
           z[f] = y[f][k];
+        } else if ((4 * f * k) % R == 0) {
+          z[f] = quarter_turns<S>(y[f][k], (4 * f * k) / R);  // W = (S i)^q exactly
         } else {
           const T c = (T)ct::cos2pi((long)f * k, R);
           const T s = (T)(S * ct::sin2pi((long)f * k, R));
