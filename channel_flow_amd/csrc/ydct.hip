@@ -31,8 +31,10 @@
 // complex Fourier coefficients as they are.  O(N log N) work per column
 // instead of the cos-table products' O(N^2).
 //
-// The FP64 form runs them in FP64 arithmetic, the mixed form in FP32 (FP64
-// coefficients in and out, FP32 point values).
+// The padded synthesis and the analysis run in FP64 arithmetic in the FP64
+// form and in FP32 in the mixed form (FP64 coefficients in and out, FP32
+// padded point values); the Gauss-Lobatto point values and the right-hand
+// sides are FP64 arithmetic in both forms.
 //
 // Layout: a block owns TX adjacent complex columns (a row of a block's
 // columns is one 128-byte line of the arithmetic type), and each column's line
