@@ -526,7 +526,6 @@ int build(Ctx* c) {
   CHF_TRY(hipMemcpy(c->tpts, tp.data(), tp.size() * sizeof(double), hipMemcpyHostToDevice));
 
   // contraction operators packed for the matrix cores (even/odd degree split)
-  const int Ny = d.Ny, ne = d.ne, no = d.no;
   {
     std::vector<double> ct(4 * (size_t)N);
     for (int n = 0; n < 4 * N; ++n) ct[n] = cospi_frac(n, 2L * N);
