@@ -151,15 +151,16 @@ def test_resume_is_bit_identical(require_gpu, precision, nnz):
         np.testing.assert_array_equal(fr[k], fs[k], err_msg=f"{precision} {k}")
 
 
-def test_h_point_values_whatever_the_call_order(require_gpu, monkeypatch):
+@pytest.mark.parametrize("nnz", [32, 96])
+def test_h_point_values_whatever_the_call_order(require_gpu, monkeypatch, nnz):
     """With the FFT transforms the next solve's right-hand-side kernels form
     h^n's point values, or chf_get_state or a second nonlinear pass forms them
     first.  (1) A run split by state reads and separate solve / nonlinear calls
     is bit-identical to chf_step straight through.  (2) Two nonlinear passes in
     a row (the AB2 history then holds the same h twice) leave the state that
     the cos-table transforms, which form the point values in the nonlinear
-    pass, leave (FP64, 1e-11)."""
-    cfg = dict(nnx=32, nny=32, nnz=32, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0, dt=1.0e-3,
+    pass, leave (FP64, 1e-11).  N = 32 and 96: FFT splits 8 x 8 and 12 x 16."""
+    cfg = dict(nnx=32, nny=32, nnz=nnz, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0, dt=1.0e-3,
                u_c=1.0, rms=0.1, kmax=8.0, slope=-4.0 / 3.0, seed=43)
     arrays = ic_of(cfg)
     monkeypatch.delenv("CHF_Y_TRANSFORMS", raising=False)
