@@ -144,6 +144,7 @@ void free_all(Ctx* c) {
   };
   f(c->alpha); f(c->beta);
   f(c->v); f(c->vst); f(c->oy); f(c->u); f(c->w); f(c->ox); f(c->oz);
+  f(c->uf); f(c->wf); f(c->oxf); f(c->ozf);
   for (int i = 0; i < 2; ++i) { f(c->hv[i]); f(c->hg[i]); f(c->cl1[i]); f(c->cl3[i]); }
   f(c->au0); f(c->w0);
   for (auto& y : c->Y) f(y);
@@ -573,6 +574,15 @@ int build(Ctx* c) {
     for (int k = 0; k < N; ++k) ty.push_back(make_double2(cospi_frac(k, 2L * N), cospi_frac(N - k, 2L * N)));
     CHF_TRY(dalloc(c, &c->twy, ty.size()));
     CHF_TRY(hipMemcpy(c->twy, ty.data(), ty.size() * sizeof(double2), hipMemcpyHostToDevice));
+    if (c->prm.precision == 1) {
+      // u, w, omega_x, omega_z as the padded FP32 synthesis reads them; the
+      // span Nyquist column is never written and stays zero
+      for (float2** q : {&c->uf, &c->wf, &c->oxf, &c->ozf}) {
+        CHF_TRY(dalloc(c, q, fld));
+        CHF_TRY(hipMemsetAsync(*q, 0, fld * sizeof(float2), c->stream));
+      }
+      c->f32_fields = true;
+    }
   }
 
   // twiddles of the unpadded physical transform (lengths nnx and nny)
@@ -653,6 +663,32 @@ hipError_t on_two_streams(Ctx* c, A&& a, B&& b) {
   return e;
 }
 
+// Mixed form: the padded FP32 synthesis of fields f0 .. f0 + nf - 1 of
+// (u, v, w, omega_x, omega_y, omega_z) into Yf.  u, w, omega_x, omega_z come
+// from their FP32 copies when those are current: the values the FP64 inputs
+// are rounded to on load, so the result is the same either way.
+hipError_t synth32_fields(Ctx* c, const double2* const* in6, int f0, int nf) {
+  if (!c->f32_current) return launch_synth_padded32(*c, in6 + f0, c->Yf + f0, nf);
+  const float2* f32src[6] = {c->uf, nullptr, c->wf, c->oxf, nullptr, c->ozf};
+  const float2* fin[6];
+  const double2* din[6];
+  float2* fout[6];
+  float2* dout[6];
+  int nfin = 0, ndin = 0;
+  for (int f = f0; f < f0 + nf; ++f) {
+    if (f32src[f]) {
+      fin[nfin] = f32src[f];
+      fout[nfin++] = c->Yf[f];
+    } else {
+      din[ndin] = in6[f];
+      dout[ndin++] = c->Yf[f];
+    }
+  }
+  hipError_t e = nfin ? launch_synth_padded32f(*c, fin, fout, nfin) : hipSuccess;
+  if (e == hipSuccess && ndin) e = launch_synth_padded32(*c, din, dout, ndin);
+  return e;
+}
+
 // defer: with the FFT transforms, leave h^n's point values to the next
 // solve's right-hand-side kernels (the ensemble's graphs form them here).
 int nonlinear(Ctx* c, bool defer = true) {
@@ -676,8 +712,8 @@ int nonlinear(Ctx* c, bool defer = true) {
       const float2* z3[3] = {c->Zf[0], c->Zf[1], c->Zf[2]};
       const float2* y3[3] = {c->Yf[0], c->Yf[1], c->Yf[2]};
       CHF_TRY(on_two_streams(
-          c, [&] { const hipError_t e = launch_synth_padded32(*c, in6, c->Yf, 3); return e ? e : launch_zfft32(*c, y6, c->Zf, 3, true); },
-          [&] { const hipError_t e = launch_synth_padded32(*c, in6 + 3, c->Yf + 3, 3); return e ? e : launch_zfft32(*c, y6 + 3, c->Zf + 3, 3, true); }));
+          c, [&] { const hipError_t e = synth32_fields(c, in6, 0, 3); return e ? e : launch_zfft32(*c, y6, c->Zf, 3, true); },
+          [&] { const hipError_t e = synth32_fields(c, in6, 3, 3); return e ? e : launch_zfft32(*c, y6 + 3, c->Zf + 3, 3, true); }));
       CHF_TRY(launch_xpass32(*c));
       CHF_TRY(on_two_streams(
           c, [&] { const hipError_t e = launch_zfft32(*c, z3, c->Yf, 2, false); return e ? e : launch_analysis32(*c, y3, c->Lc, 2); },
@@ -695,7 +731,7 @@ int nonlinear(Ctx* c, bool defer = true) {
     const float2* y6[6] = {c->Yf[0], c->Yf[1], c->Yf[2], c->Yf[3], c->Yf[4], c->Yf[5]};
     const float2* z3[3] = {c->Zf[0], c->Zf[1], c->Zf[2]};
     const float2* y3[3] = {c->Yf[0], c->Yf[1], c->Yf[2]};
-    CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return launch_synth_padded32(*c, in6, c->Yf, 6); }));
+    CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return synth32_fields(c, in6, 0, 6); }));
     CHF_TRY(timed(c, K_ZFFT_INV, [&] { return launch_zfft32(*c, y6, c->Zf, 6, true); }));
     CHF_TRY(timed(c, K_XPASS, [&] { return launch_xpass32(*c); }));
     CHF_TRY(timed(c, K_ZFFT_FWD, [&] { return launch_zfft32(*c, z3, c->Yf, 3, false); }));
@@ -740,7 +776,11 @@ int solve(Ctx* c) {
     CHF_TRY(timed(c, K_SOLVE, [&] { return launch_solve(*c); }));
     if (c->wdiag) CHF_TRY(timed(c, K_WSOLVE, [&] { return launch_wsolve(*c); }));
   }
-  CHF_TRY(timed(c, K_KINEMATICS, [&] { return launch_kinematics(*c); }));
+  // the mixed form with the FFT transforms keeps u, w, omega_x, omega_z in
+  // FP32 only, as the next nonlinear pass reads them
+  CHF_TRY(timed(c, K_KINEMATICS, [&] { return launch_kinematics(*c, c->f32_fields); }));
+  c->f32_current = c->f32_fields;
+  c->f64_stale = c->f32_fields;
   return 0;
 }
 
@@ -759,6 +799,8 @@ int solve_before(Ctx* c) {
 int solve_after(Ctx* c) {
   CHF_TRY(launch_vtranspose(*c));
   CHF_TRY(timed(c, K_KINEMATICS, [&] { return launch_kinematics(*c); }));
+  c->f32_current = false;
+  c->f64_stale = false;
   return 0;
 }
 
@@ -781,7 +823,19 @@ int check_finite(Ctx* c, const char* who) {
 hipError_t load_kinematics(Ctx* c) {
   hipError_t e = hipMemsetAsync(c->nonfinite, 0, sizeof(int), c->stream);
   if (e != hipSuccess) return e;
+  c->f32_current = false;
+  c->f64_stale = false;
   return launch_kinematics(*c);
+}
+
+// u, w, omega_x, omega_z in FP64 for the calls that read them, when the step
+// kept them in FP32 only: the same kinematics pass with FP64 outputs (it also
+// rewrites the brackets with the values they hold).
+hipError_t form_fields64(Ctx* c) {
+  if (!c->f64_stale) return hipSuccess;
+  const hipError_t e = launch_kinematics(*c);
+  if (e == hipSuccess) c->f64_stale = false;
+  return e;
 }
 
 }  // namespace
@@ -927,6 +981,7 @@ int chf_get_fields(void* ctx, double* unp1, double* wnp1, double* omegaxnp1, dou
   if (!ctx) return -1;
   Ctx* c = static_cast<Ctx*>(ctx);
   const size_t fb = (size_t)c->d.Ny * c->d.ncol * sizeof(double2);
+  CHF_TRY(form_fields64(c));
   struct { const double2* src; double* dst; } m[] = {
       {c->u, unp1}, {c->w, wnp1}, {c->ox, omegaxnp1}, {c->oz, omegaznp1}};
   for (auto& e : m)
@@ -999,6 +1054,7 @@ int chf_physical(void* ctx, int field, double* out) {
   if (!out) return fail(c, "chf_physical: null output");
   const int rc = ensure_physical(c);
   if (rc) return rc;
+  CHF_TRY(form_fields64(c));
   CHF_TRY(launch_physical(*c, f, c->phys_c, c->phys_r[0]));
   const size_t n = (size_t)c->d.Ny * c->d.Mz * 2 * c->d.Mx;
   CHF_TRY(hipMemcpyAsync(out, c->phys_r[0], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1017,6 +1073,7 @@ int chf_physical_noalloc(void* ctx, int field, double* out) {
     return fail(c, "chf_physical_noalloc: the transform needs " + std::to_string(need) +
                        " bytes of work memory (Ny * nny * nnx/2 complex values), more than its " +
                        std::to_string(kScratchBytes) + "-byte scratch; use chf_physical");
+  CHF_TRY(form_fields64(c));
   CHF_TRY(launch_physical(*c, f, c->scratch, out));
   return 0;
 }
@@ -1027,6 +1084,7 @@ int chf_statistics(void* ctx, double* out) {
   if (!out) return fail(c, "chf_statistics: null output");
   const int rc = ensure_physical(c);
   if (rc) return rc;
+  CHF_TRY(form_fields64(c));
   for (int q = 0; q < 3; ++q) CHF_TRY(launch_physical(*c, field_ptr(c, q), c->phys_c, c->phys_r[q]));
   CHF_TRY(launch_plane_stats(*c, c->phys_r[0], c->phys_r[1], c->phys_r[2], c->stats));
   CHF_TRY(hipMemcpyAsync(out, c->stats, 12 * (size_t)c->d.Ny * sizeof(double), hipMemcpyDeviceToHost,

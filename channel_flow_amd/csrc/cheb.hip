@@ -177,6 +177,11 @@ hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* cons
   return synth_t<double, double, float>(c, c.costabf, 2, c.d.Pn, c.d.Ny, ri, ro, nf, c.cb_choice[K_SYNTH_POINTS]);
 }
 
+hipError_t launch_synth_padded32f(Ctx& c, const float2* const* in, float2* const* out, int nf) {
+  if (!c.twy) return hipErrorNotSupported;
+  return launch_ysynth32f(c, in, out, nf);
+}
+
 // Mixed form: FP64 coefficients -> FP32 values on the padded grid, FP32 matrix cores.
 hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const* out, int nf) {
   if (c.twy) return launch_ysynth32(c, in, out, nf);

@@ -101,6 +101,12 @@ struct Ctx {
   double2 *v = nullptr, *oy = nullptr;              // coefficients
   double2* vst = nullptr;  // v as the solve writes it: per (kx, |kz|) group, [Ny][2 sides]
   double2 *u = nullptr, *w = nullptr, *ox = nullptr, *oz = nullptr;  // derived
+  // Mixed form with the FFT transforms: u, w, omega_x, omega_z also in FP32,
+  // the values the padded synthesis rounds them to on load.  The per-step
+  // kinematics writes only these (f64_stale: u, w, ox, oz are not current and
+  // are formed from v, omega_y, U, W when read); f32_current: they are.
+  float2 *uf = nullptr, *wf = nullptr, *oxf = nullptr, *ozf = nullptr;
+  bool f32_fields = false, f32_current = false, f64_stale = false;
   double2* hv[2] = {nullptr, nullptr};  // h_v point values; [hcur] = h^n
   double2* hg[2] = {nullptr, nullptr};  // h_g point values
   double2* cl1[2] = {nullptr, nullptr};  // mean L_x(0,0) point values [Ny]
@@ -207,6 +213,8 @@ bool reg_fft_length32(int L);  // register span FFTs; FP32 streamwise pass
 hipError_t launch_zfft32(Ctx& c, const float2* const* in, float2* const* out, int nf, bool inv);
 hipError_t launch_xpass32(Ctx& c);
 hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const* out, int nf);
+// FP32 coefficients in (FFT transforms only): the values launch_synth_padded32 rounds its inputs to
+hipError_t launch_synth_padded32f(Ctx& c, const float2* const* in, float2* const* out, int nf);
 hipError_t launch_analysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_zfft_inv(Ctx& c, double2* const* in, double2* const* out, int nf);
 hipError_t launch_xpass(Ctx& c);
@@ -225,6 +233,7 @@ hipError_t launch_yanalysis(Ctx& c, const double2* const* in, double2* const* ou
 hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hc, double2* hn, const double2* hp,
                        double2* oE, double2* oO, double2* oT);
 hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf);
+hipError_t launch_ysynth32f(Ctx& c, const float2* const* in, float2* const* out, int nf);  // FP32 coefficients in
 hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf);  // FP32 compute in the mixed form
 hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);
@@ -242,7 +251,9 @@ size_t solve_members_bytes(int n);
 hipError_t write_solve_members(Ctx* const* m, int n, void* dev);
 hipError_t launch_solve_members(hipStream_t st, Ctx* const* m, int n, const void* dev);
 hipError_t launch_wsolve(Ctx& c);      // diagonalised omega_y solve (all modes but the mean)
-hipError_t launch_kinematics(Ctx& c);  // u, w, omega_x, omega_z (and the next RHS brackets) from v, omega_y, U, W
+// u, w, omega_x, omega_z (and the next RHS brackets) from v, omega_y, U, W;
+// f32: u, w, omega_x, omega_z into uf, wf, oxf, ozf (rounded to FP32) instead
+hipError_t launch_kinematics(Ctx& c, bool f32 = false);
 hipError_t launch_setup_inverses(Ctx& c);
 
 }  // namespace chf

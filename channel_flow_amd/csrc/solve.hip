@@ -453,6 +453,7 @@ __global__ __launch_bounds__(512) void k_solve(SolveArgs a1, const SolveArgs* __
 struct KinArgs {
   const double2 *v, *oy, *au0, *w0;
   double2 *u, *w, *ox, *oz;
+  float2 *uf, *wf, *oxf, *ozf;  // F32: u, w, omega_x, omega_z rounded to FP32, written instead
   double2 *gv, *go;  // brackets for the next solve
   const double *al, *be;
   int* nonfinite;    // set to 1 when v, omega_y, U or W hold a NaN or an infinity
@@ -461,6 +462,19 @@ struct KinArgs {
   double c;          // dt / (2 re)
 };
 
+// u, w, omega_x, omega_z at one offset: FP64, or (F32) rounded to FP32
+template <bool F32>
+__device__ __forceinline__ void kin_store(const KinArgs& a, int which, long off, double2 v) {
+  if constexpr (F32) {
+    float2* p = which == 0 ? a.uf : which == 1 ? a.wf : which == 2 ? a.oxf : a.ozf;
+    st_nt(p + off, make_float2((float)v.x, (float)v.y));
+  } else {
+    double2* p = which == 0 ? a.u : which == 1 ? a.w : which == 2 ? a.ox : a.oz;
+    st_nt(p + off, v);
+  }
+}
+
+template <bool F32>
 __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
   const int q = blockIdx.y;  // chain: parity of the v degrees it reads
   const int N = a.N;
@@ -493,10 +507,10 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
     __syncthreads();
     for (int k = threadIdx.x; k < a.Ny; k += blockDim.x) {
       const long off = (long)k * a.ncol;
-      a.u[off] = make_double2(x[0 * a.Ny + k], x[1 * a.Ny + k]);
-      a.w[off] = make_double2(x[2 * a.Ny + k], x[3 * a.Ny + k]);
-      a.oz[off] = make_double2(-1.0 * d[0 * a.Ny + k], -1.0 * d[1 * a.Ny + k]);
-      a.ox[off] = make_double2(d[2 * a.Ny + k], d[3 * a.Ny + k]);
+      kin_store<F32>(a, 0, off, make_double2(x[0 * a.Ny + k], x[1 * a.Ny + k]));
+      kin_store<F32>(a, 1, off, make_double2(x[2 * a.Ny + k], x[3 * a.Ny + k]));
+      kin_store<F32>(a, 3, off, make_double2(-1.0 * d[0 * a.Ny + k], -1.0 * d[1 * a.Ny + k]));
+      kin_store<F32>(a, 2, off, make_double2(d[2 * a.Ny + k], d[3 * a.Ny + k]));
     }
     return;
   }
@@ -541,8 +555,8 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
           du.x *= 0.5; du.y *= 0.5; dw.x *= 0.5; dw.y *= 0.5;
         }
         const double2 vk = in[s];
-        st_nt(&a.ox[off], make_double2(dw.x + be * vk.y, dw.y - be * vk.x));
-        st_nt(&a.oz[off], make_double2(-du.x - al * vk.y, -du.y + al * vk.x));
+        kin_store<F32>(a, 2, off, make_double2(dw.x + be * vk.y, dw.y - be * vk.x));
+        kin_store<F32>(a, 3, off, make_double2(-du.x - al * vk.y, -du.y + al * vk.x));
         // brackets: D^2 v, D^4 v at k from D v, D^3 v at k+1; D omega at k from omega_{k+1}
         double2 q2 = make_double2(d2s.x + f * dv2.x, d2s.y + f * dv2.y);
         double2 q4 = make_double2(d4s.x + f * d3s.x, d4s.y + f * d3s.y);
@@ -564,8 +578,8 @@ __global__ __launch_bounds__(256) void k_kinematics(KinArgs a) {
         const double omr = in[s].x, omi = in[s].y;
         const double2 uk = make_double2(-1.0 * al * dv.y / k2 + be * omi / k2, al * dv.x / k2 - be * omr / k2);
         const double2 wk = make_double2(-1.0 * dv.y * be / k2 - 1.0 * omi * al / k2, be * dv.x / k2 + al * omr / k2);
-        st_nt(&a.u[off], uk);
-        st_nt(&a.w[off], wk);
+        kin_store<F32>(a, 0, off, uk);
+        kin_store<F32>(a, 1, off, wk);
         up = uk;
         wp = wk;
         // brackets: D^3 v at k from D^2 v at k+1; D^2 omega at k from D omega at k+1
@@ -918,9 +932,14 @@ hipError_t launch_wsolve(Ctx& c) {
   return hipGetLastError();
 }
 
-hipError_t launch_kinematics(Ctx& c) {
+hipError_t launch_kinematics(Ctx& c, bool f32) {
+  if (f32 && !c.uf) return hipErrorInvalidValue;
   const Dims& d = c.d;
   KinArgs a{};
+  a.uf = c.uf;
+  a.wf = c.wf;
+  a.oxf = c.oxf;
+  a.ozf = c.ozf;
   a.v = c.v;
   a.oy = c.oy;
   a.au0 = c.au0;
@@ -942,12 +961,15 @@ hipError_t launch_kinematics(Ctx& c) {
   a.c = c.prm.dt * 0.5 / c.prm.re;
   const unsigned nb = (unsigned)((d.ncol + kColumnThreads - 1) / kColumnThreads) + 1;  // + the mean-mode block
   const size_t shm = sizeof(double) * 8ull * d.Ny;
+  const void* k = f32 ? (const void*)k_kinematics<true> : (const void*)k_kinematics<false>;
   if (shm > 65536) {
-    hipError_t e =
-        hipFuncSetAttribute((const void*)k_kinematics, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_kinematics, dim3(nb, 2), dim3(kColumnThreads), shm, c.stream, a);
+  if (f32)
+    hipLaunchKernelGGL(k_kinematics<true>, dim3(nb, 2), dim3(kColumnThreads), shm, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_kinematics<false>, dim3(nb, 2), dim3(kColumnThreads), shm, c.stream, a);
   return hipGetLastError();
 }
 
