@@ -60,7 +60,9 @@ def kernel_work(d, precision="fp64", yfft=False):
     O(N log N) work per column and are priced by their bytes alone, and the
     right-hand-side kernels form h^n's point values themselves (they read
     h^n's coefficients and h^(n-1)'s point values and write h^n's): there is
-    no separate point-value pass.
+    no separate point-value pass.  In the mixed form with yfft the kinematics
+    writes u, w, omega_x, omega_z in FP32 and the padded synthesis reads them
+    so (v and omega_y stay FP64).
     """
     w = 16 if precision == "fp64" else 8
     S = d["Ny"] * d["ncol"] * 16            # spectral field (FP64)
@@ -73,8 +75,10 @@ def kernel_work(d, precision="fp64", yfft=False):
     inv_v_bytes = (d["nclass"] - 1) * (ne * ne + no * no) * 8   # one per k^2 class; k^2 = 0 has no v system
     ws_flops = 2.0 * col * ((ne - 1) * (ne - 1) + ne * (ne - 1) + (no - 1) * (no - 1) + no * (no - 1))
     yf = 0.0 if yfft else 1.0
+    f32f = precision != "fp64" and yfft      # u, w, omega_x, omega_z held in FP32
+    S_in = (4 * S / 2 + 2 * S) if f32f else 6 * S
     return {
-        "synth_padded": (6 * (S + Y), yf * 2.0 * Ph * Ny * col * 6),
+        "synth_padded": (S_in + 6 * Y, yf * 2.0 * Ph * Ny * col * 6),
         "zfft_inv": (6 * (Y + Z), 0.0),
         "xpass": (9 * Z, 0.0),
         "zfft_fwd": (3 * (Z + Y), 0.0),
@@ -88,7 +92,8 @@ def kernel_work(d, precision="fp64", yfft=False):
         # v solve (omega_y of the mean group only) and the diagonalised omega_y solve
         "solve": (inv_v_bytes + rv_rows / Ny * S + S, 0.0),
         "wsolve": (ro_rows / Ny * S + S, ws_flops),
-        "kinematics": (8 * S, 0.0),   # reads v, omega_y; writes u, w, omega_x, omega_z and the two brackets
+        # reads v, omega_y; writes u, w, omega_x, omega_z and the two brackets
+        "kinematics": ((2 * S + 4 * S / 2 + 2 * S) if f32f else 8 * S, 0.0),
     }
 
 

@@ -108,7 +108,10 @@ int chf_sync(void* ctx);
 
 /* Post-solve velocity and vorticity (unp1, wnp1, omegaxnp1, omegaznp1 as the
  * reference holds them after its solve loop, PF:387-472), any pointer may be
- * NULL.  Valid after chf_solve or chf_init_fields. */
+ * NULL.  Valid after chf_solve or chf_init_fields.  The mixed form with the
+ * FFT transforms keeps these four fields in FP32 during a run (the values its
+ * padded synthesis reads); this call, chf_physical and chf_statistics first
+ * form them in FP64 from v, omega_y, U, W (one kinematics pass). */
 int chf_get_fields(void* ctx, double* unp1, double* wnp1, double* omegaxnp1,
                    double* omegaznp1);
 
