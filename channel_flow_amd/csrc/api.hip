@@ -145,6 +145,7 @@ void free_all(Ctx* c) {
   f(c->alpha); f(c->beta);
   f(c->v); f(c->vst); f(c->oy); f(c->u); f(c->w); f(c->ox); f(c->oz);
   f(c->uf); f(c->wf); f(c->oxf); f(c->ozf);
+  for (auto& l : c->Lcf) f(l);
   for (int i = 0; i < 2; ++i) { f(c->hv[i]); f(c->hg[i]); f(c->cl1[i]); f(c->cl3[i]); }
   f(c->au0); f(c->w0);
   for (auto& y : c->Y) f(y);
@@ -581,6 +582,10 @@ int build(Ctx* c) {
         CHF_TRY(dalloc(c, q, fld));
         CHF_TRY(hipMemsetAsync(*q, 0, fld * sizeof(float2), c->stream));
       }
+      for (auto& l : c->Lcf) {
+        CHF_TRY(dalloc(c, &l, fld));
+        CHF_TRY(hipMemsetAsync(l, 0, fld * sizeof(float2), c->stream));
+      }
       c->f32_fields = true;
     }
   }
@@ -689,6 +694,14 @@ hipError_t synth32_fields(Ctx* c, const double2* const* in6, int f0, int nf) {
   return e;
 }
 
+// Mixed form: the FP32 analysis of L_x, L_y, L_z (f0 .. f0 + nf - 1) into
+// Lcf (FP32, with the FFT transforms: the same values) or Lc, whichever the
+// h assembly reads.
+hipError_t analysis32_fields(Ctx* c, const float2* const* y3, int f0, int nf) {
+  if (c->f32_fields) return launch_analysis32f(*c, y3 + f0, c->Lcf + f0, nf);
+  return launch_analysis32(*c, y3 + f0, c->Lc + f0, nf);
+}
+
 // defer: with the FFT transforms, leave h^n's point values to the next
 // solve's right-hand-side kernels (the ensemble's graphs form them here).
 int nonlinear(Ctx* c, bool defer = true) {
@@ -716,8 +729,8 @@ int nonlinear(Ctx* c, bool defer = true) {
           [&] { const hipError_t e = synth32_fields(c, in6, 3, 3); return e ? e : launch_zfft32(*c, y6 + 3, c->Zf + 3, 3, true); }));
       CHF_TRY(launch_xpass32(*c));
       CHF_TRY(on_two_streams(
-          c, [&] { const hipError_t e = launch_zfft32(*c, z3, c->Yf, 2, false); return e ? e : launch_analysis32(*c, y3, c->Lc, 2); },
-          [&] { const hipError_t e = launch_zfft32(*c, z3 + 2, c->Yf + 2, 1, false); return e ? e : launch_analysis32(*c, y3 + 2, c->Lc + 2, 1); }));
+          c, [&] { const hipError_t e = launch_zfft32(*c, z3, c->Yf, 2, false); return e ? e : analysis32_fields(c, y3, 0, 2); },
+          [&] { const hipError_t e = launch_zfft32(*c, z3 + 2, c->Yf + 2, 1, false); return e ? e : analysis32_fields(c, y3, 2, 1); }));
     }
   } else if (c->prm.precision == 0) {
     CHF_TRY(timed(c, K_SYNTH_PADDED, [&] { return launch_synth_padded(*c, in6, c->Y, 6); }));
@@ -735,7 +748,7 @@ int nonlinear(Ctx* c, bool defer = true) {
     CHF_TRY(timed(c, K_ZFFT_INV, [&] { return launch_zfft32(*c, y6, c->Zf, 6, true); }));
     CHF_TRY(timed(c, K_XPASS, [&] { return launch_xpass32(*c); }));
     CHF_TRY(timed(c, K_ZFFT_FWD, [&] { return launch_zfft32(*c, z3, c->Yf, 3, false); }));
-    CHF_TRY(timed(c, K_ANALYSIS, [&] { return launch_analysis32(*c, y3, c->Lc, 3); }));
+    CHF_TRY(timed(c, K_ANALYSIS, [&] { return analysis32_fields(c, y3, 0, 3); }));
   }
   CHF_TRY(timed(c, K_HASSEMBLE, [&] { return launch_hassemble(*c); }));
   const int nxt = 1 - c->hcur;

@@ -177,6 +177,11 @@ hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* cons
   return synth_t<double, double, float>(c, c.costabf, 2, c.d.Pn, c.d.Ny, ri, ro, nf, c.cb_choice[K_SYNTH_POINTS]);
 }
 
+hipError_t launch_analysis32f(Ctx& c, const float2* const* in, float2* const* out, int nf) {
+  if (!c.twy) return hipErrorNotSupported;
+  return launch_yanalysis32f(c, in, out, nf);
+}
+
 hipError_t launch_synth_padded32f(Ctx& c, const float2* const* in, float2* const* out, int nf) {
   if (!c.twy) return hipErrorNotSupported;
   return launch_ysynth32f(c, in, out, nf);
@@ -419,19 +424,44 @@ hipError_t launch_rhs(Ctx& c, int which) {
 
 // ---------------------------------------------------------------- h assembly
 
+// L^n coefficients in FP64, or (mixed form with FFT transforms) in FP32: the
+// FP32 analysis's results, exact in either width
+struct LIn {
+  const double2* d;
+  const float2* f;
+  // the stored value (float2 or double2); widened where it is used, so that
+  // a batch of loads stays in flight
+  template <bool F32>
+  __device__ __forceinline__ auto raw(long i) const {
+    if constexpr (F32) return f[i];
+    else return d[i];
+  }
+  template <bool F32>
+  __device__ __forceinline__ double2 at(long i) const {
+    const auto v = raw<F32>(i);
+    return make_double2(v.x, v.y);
+  }
+};
+
+template <class V>
+__device__ __forceinline__ double2 widen(V v) {
+  return make_double2(v.x, v.y);
+}
+
 struct MArgs {
-  const double2 *lx, *lz;
+  LIn lx, lz;
   double2 *o1, *o3;
   const double* tpts;  // [Pn][Ny]
   int N, Ny, Pn;
   long ncol;
 };
 
+template <bool F32>
 __device__ void mean_points_block(const MArgs& a) {
   __shared__ double2 s1[260], s3[260];
   for (int l = threadIdx.x; l <= a.N; l += blockDim.x) {
-    s1[l] = a.lx[(long)l * a.ncol];
-    s3[l] = a.lz[(long)l * a.ncol];
+    s1[l] = a.lx.at<F32>((long)l * a.ncol);
+    s3[l] = a.lz.at<F32>((long)l * a.ncol);
   }
   __syncthreads();
   for (int jj = threadIdx.x; jj < a.Pn; jj += blockDim.x) {
@@ -460,7 +490,7 @@ __device__ void mean_points_block(const MArgs& a) {
 
 
 struct HArgs {
-  const double2 *lx, *ly, *lz;
+  LIn lx, ly, lz;
   double2 *hv, *hg;
   const double *al, *be;
   int Mx, N;
@@ -472,9 +502,10 @@ struct HArgs {
 // sum b_k = (2/c_k) sum_{j>k, j+k odd} j a_j.
 // The launch carries one extra block that computes the mean-mode Lamb point
 // values (mean_points_block) from the same coefficients.
+template <bool F32>
 __global__ __launch_bounds__(256) void k_hassemble(HArgs a, MArgs m) {
   if (blockIdx.x == gridDim.x - 1) {
-    mean_points_block(m);
+    mean_points_block<F32>(m);
     return;
   }
   const long cm = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -489,15 +520,15 @@ __global__ __launch_bounds__(256) void k_hassemble(HArgs a, MArgs m) {
   double2 lxn = make_double2(0, 0), lzn = make_double2(0, 0);  // a_{k+1}
   for (int k0 = N; k0 >= 0; k0 -= 4) {
     // loads issued four degrees ahead of the recurrence
-    double2 pl[4][3];
+    decltype(a.lx.raw<F32>(0)) pl[4][3];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int k = k0 - t;
       if (k >= 0) {
         const long off = (long)k * a.ncol + cm;
-        pl[t][0] = a.lx[off];
-        pl[t][1] = a.ly[off];
-        pl[t][2] = a.lz[off];
+        pl[t][0] = a.lx.raw<F32>(off);
+        pl[t][1] = a.ly.raw<F32>(off);
+        pl[t][2] = a.lz.raw<F32>(off);
       }
     }
 #pragma unroll
@@ -505,7 +536,7 @@ __global__ __launch_bounds__(256) void k_hassemble(HArgs a, MArgs m) {
     const int k = k0 - t;
     if (k < 0) break;
     const long off = (long)k * a.ncol + cm;
-    const double2 lx = pl[t][0], ly = pl[t][1], lz = pl[t][2];
+    const double2 lx = widen(pl[t][0]), ly = widen(pl[t][1]), lz = widen(pl[t][2]);
     double2 dx, dz;
     if (k == N) {
       dx = make_double2(0, 0);
@@ -543,9 +574,10 @@ __global__ __launch_bounds__(256) void k_hassemble(HArgs a, MArgs m) {
 
 hipError_t launch_hassemble(Ctx& c) {
   HArgs a{};
-  a.lx = c.Lc[0];
-  a.ly = c.Lc[1];
-  a.lz = c.Lc[2];
+  const bool f32 = c.f32_fields;  // the analysis wrote L^n in FP32 (Lcf)
+  a.lx = LIn{c.Lc[0], f32 ? c.Lcf[0] : nullptr};
+  a.ly = LIn{c.Lc[1], f32 ? c.Lcf[1] : nullptr};
+  a.lz = LIn{c.Lc[2], f32 ? c.Lcf[2] : nullptr};
   a.hv = c.hvc;
   a.hg = c.hgc;
   a.al = c.alpha;
@@ -554,8 +586,8 @@ hipError_t launch_hassemble(Ctx& c) {
   a.N = c.d.N;
   a.ncol = c.d.ncol;
   MArgs m{};
-  m.lx = c.Lc[0];
-  m.lz = c.Lc[2];
+  m.lx = a.lx;
+  m.lz = a.lz;
   m.o1 = c.cl1[1 - c.hcur];
   m.o3 = c.cl3[1 - c.hcur];
   m.tpts = c.tpts;
@@ -563,8 +595,11 @@ hipError_t launch_hassemble(Ctx& c) {
   m.Ny = c.d.Ny;
   m.Pn = c.d.Pn;
   m.ncol = c.d.ncol;
-  hipLaunchKernelGGL(k_hassemble, dim3((unsigned)((a.ncol + kColumnThreads - 1) / kColumnThreads) + 1),
-                     dim3(kColumnThreads), 0, c.stream, a, m);
+  const dim3 grid((unsigned)((a.ncol + kColumnThreads - 1) / kColumnThreads) + 1);
+  if (f32)
+    hipLaunchKernelGGL(k_hassemble<true>, grid, dim3(kColumnThreads), 0, c.stream, a, m);
+  else
+    hipLaunchKernelGGL(k_hassemble<false>, grid, dim3(kColumnThreads), 0, c.stream, a, m);
   return hipGetLastError();
 }
 

@@ -106,6 +106,9 @@ struct Ctx {
   // kinematics writes only these (f64_stale: u, w, ox, oz are not current and
   // are formed from v, omega_y, U, W when read); f32_current: they are.
   float2 *uf = nullptr, *wf = nullptr, *oxf = nullptr, *ozf = nullptr;
+  // the same forms keep L^n's coefficients (the FP32 analysis's results, exact
+  // in FP32) in FP32 between the analysis and the h assembly
+  float2* Lcf[3] = {};
   bool f32_fields = false, f32_current = false, f64_stale = false;
   double2* hv[2] = {nullptr, nullptr};  // h_v point values; [hcur] = h^n
   double2* hg[2] = {nullptr, nullptr};  // h_g point values
@@ -215,6 +218,8 @@ hipError_t launch_xpass32(Ctx& c);
 hipError_t launch_synth_padded32(Ctx& c, const double2* const* in, float2* const* out, int nf);
 // FP32 coefficients in (FFT transforms only): the values launch_synth_padded32 rounds its inputs to
 hipError_t launch_synth_padded32f(Ctx& c, const float2* const* in, float2* const* out, int nf);
+// FP32 coefficients out (FFT transforms only): launch_analysis32's values, which are FP32 numbers
+hipError_t launch_analysis32f(Ctx& c, const float2* const* in, float2* const* out, int nf);
 hipError_t launch_analysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_zfft_inv(Ctx& c, double2* const* in, double2* const* out, int nf);
 hipError_t launch_xpass(Ctx& c);
@@ -234,6 +239,7 @@ hipError_t launch_yrhs(Ctx& c, int which, const double2* g, const double2* hc, d
                        double2* oE, double2* oO, double2* oT);
 hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out, int nf);
 hipError_t launch_ysynth32f(Ctx& c, const float2* const* in, float2* const* out, int nf);  // FP32 coefficients in
+hipError_t launch_yanalysis32f(Ctx& c, const float2* const* in, float2* const* out, int nf);  // FP32 coefficients out
 hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf);
 hipError_t launch_synth_points_h(Ctx& c, const double2* const* in, double2* const* out, int nf);  // FP32 compute in the mixed form
 hipError_t launch_span_inverse_unpadded(Ctx& c, double2* inout);

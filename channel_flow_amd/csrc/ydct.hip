@@ -568,6 +568,9 @@ hipError_t launch_ysynth32(Ctx& c, const double2* const* in, float2* const* out,
 hipError_t launch_ysynth32f(Ctx& c, const float2* const* in, float2* const* out, int nf) {
   return run_ydct<float2>(c, 1, in, out, nf);
 }
+hipError_t launch_yanalysis32f(Ctx& c, const float2* const* in, float2* const* out, int nf) {
+  return run_ydct<float2>(c, 2, in, out, nf);
+}
 hipError_t launch_yanalysis32(Ctx& c, const float2* const* in, double2* const* out, int nf) {
   return run_ydct<float2>(c, 2, in, out, nf);
 }
