@@ -62,7 +62,8 @@ def kernel_work(d, precision="fp64", yfft=False):
     h^n's coefficients and h^(n-1)'s point values and write h^n's): there is
     no separate point-value pass.  In the mixed form with yfft the kinematics
     writes u, w, omega_x, omega_z in FP32 and the padded synthesis reads them
-    so (v and omega_y stay FP64).
+    so (v and omega_y stay FP64), and the analysis writes L^n's coefficients
+    in FP32, which the h assembly reads so.
     """
     w = 16 if precision == "fp64" else 8
     S = d["Ny"] * d["ncol"] * 16            # spectral field (FP64)
@@ -82,8 +83,8 @@ def kernel_work(d, precision="fp64", yfft=False):
         "zfft_inv": (6 * (Y + Z), 0.0),
         "xpass": (9 * Z, 0.0),
         "zfft_fwd": (3 * (Z + Y), 0.0),
-        "analysis": (3 * (Y + S), yf * 2.0 * Ny * Ph * col * 3),
-        "hassemble": (5 * S + 4 * Ny * 16, 0.0),   # plus the mean-mode point block
+        "analysis": (3 * (Y + (S / 2 if f32f else S)), yf * 2.0 * Ny * Ph * col * 3),
+        "hassemble": ((3 * S / 2 if f32f else 3 * S) + 2 * S + 4 * Ny * 16, 0.0),   # plus the mean-mode point block
         "synth_points": (4 * S, yf * 2.0 * Pn * Ny * col * 2),
         "rhs_v": ((4 * S if yfft else S + 2 * (Ny - 4) / Ny * S) + rv_rows / Ny * S,
                   yf * 2.0 * ((ne - 2) * ne + (no - 2) * no) * col),
