@@ -122,6 +122,64 @@ def test_set_state_derives_the_fields(require_gpu):
         assert e < 1e-13, (k, e)
 
 
+def test_mixed_fields_read_after_steps_are_the_state_kinematics(require_gpu, monkeypatch):
+    """The mixed form with the FFT transforms keeps u, w, omega_x, omega_z in
+    FP32 between its steps and forms the FP64 fields only when they are read.
+    After steps (and again after more steps and a second read) chf_get_fields,
+    chf_physical and chf_statistics equal, bit for bit, what a new context
+    derives from the same state with chf_set_state, and the fields are the
+    oracle's kinematics of that state (1e-13).  The reads rewrite the next
+    solve's brackets with the values they hold: the run continues bit for bit
+    as one without reads."""
+    cfg = dict(nnx=32, nny=32, nnz=32, aphi=1.0, beta=0.5, re=3000.0, re_p=2.0 / 3000.0, dt=1.0e-3,
+               u_c=1.0, rms=0.1, kmax=8.0, slope=-4.0 / 3.0, seed=47)
+    p = params_of(cfg)
+    monkeypatch.delenv("CHF_Y_TRANSFORMS", raising=False)
+
+    def reads(ch):
+        return ch.fields(), {f: ch.physical(f) for f in ("u", "omega_z")}, ch.plane_statistics()
+
+    with make_channel(cfg, "mixed") as a:
+        assert "synth_padded" not in a.tile_choices(), "the FFT transforms are expected at this size"
+        a.initial(*ic_of(cfg))
+        runs = []
+        for n in (5, 3):
+            a.step(n)
+            st = a.get_state()
+            first = reads(a)
+            again = reads(a)
+            runs.append((st, first, again))
+    with make_channel(cfg, "mixed") as s:
+        s.initial(*ic_of(cfg))
+        s.step(8)
+        straight = s.get_state()
+    for k in STATE_KEYS:
+        np.testing.assert_array_equal(runs[-1][0][k], straight[k], err_msg=f"reads changed the run: {k}")
+    mz = p.Mz
+    for st, (fl, ph, ps), (fl2, ph2, ps2) in runs:
+        with make_channel(cfg, "mixed") as b:
+            b.set_state(st)
+            rf, rph, rps = reads(b)
+        for k in fl:
+            np.testing.assert_array_equal(fl[k], rf[k], err_msg=k)
+            np.testing.assert_array_equal(fl2[k], rf[k], err_msg=k)
+        for f in ph:
+            np.testing.assert_array_equal(ph[f], rph[f], err_msg=f)
+            np.testing.assert_array_equal(ph2[f], rph[f], err_msg=f)
+        np.testing.assert_array_equal(ps, rps)
+        np.testing.assert_array_equal(ps2, rps)
+        ref = {k: np.array(v, copy=True) for k, v in st.items()}
+        for k in fl:                       # the record holds no fields; every mode is set below
+            ref[k] = np.zeros_like(st["vnp1"])
+        O.kinematics_all(ref, p)
+        for k in ("unp1", "wnp1", "omegaxnp1", "omegaznp1"):
+            g, r = fl[k].copy(), ref[k].copy()
+            g[:, mz // 2, :] = 0
+            r[:, mz // 2, :] = 0
+            e = float(np.max(np.abs(g - r))) / float(np.max(np.abs(r)))
+            assert e < 1e-13, (k, e)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("nnz", [24, 32])
 def test_resume_is_bit_identical(require_gpu, precision, nnz):
